@@ -1,0 +1,243 @@
+"""Pin the oracle's sampler, contraction, compositing, top-k, MLP, full render
+backward and optimizer (render.py, networks.py, training.py) against brute force,
+closed forms, torch float64 autograd, finite differences and torch.optim.Adam."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import tensorf_oracle as orc
+from tests import torch_ref
+
+RNG = np.random.default_rng(7)
+BOX = np.array([[-1.5, -1.5, -1.5], [1.5, 1.5, 1.5]])
+
+
+def _inside(p, box, tol=1e-9):
+    return np.all((p >= box[0] - tol) & (p <= box[1] + tol), axis=-1)
+
+
+def test_bbox_segment_brute_force():
+    o, d = orc.make_rays(300, seed=3, radius=4.0, target_half=2.5)  # some rays miss
+    t0, t1 = orc.ray_segment_from_bounding_box(o, d, BOX)
+    ts = np.linspace(0, 12, 24001)
+    for r in range(o.shape[0]):
+        pts = o[r] + ts[:, None] * d[r]
+        ins = _inside(pts, BOX)
+        if ins.any():
+            lo, hi = ts[ins].min(), ts[ins].max()
+            assert abs(t0[r] - lo) < 1e-3 and abs(t1[r] - hi) < 1e-3
+            assert _inside(o[r] + t0[r] * d[r], BOX, 1e-7)
+            assert _inside(o[r] + t1[r] * d[r], BOX, 1e-7)
+        else:  # misses -> [0, 1e-3]
+            assert t0[r] == 0.0 and t1[r] == 1e-3
+    # origin inside the box: t_min clamps to 0
+    t0, t1 = orc.ray_segment_from_bounding_box(np.zeros((1, 3)), np.array([[0.0, 0.6, 0.8]]), BOX)
+    assert t0[0] == 0.0 and abs(t1[0] - 1.5 / 0.8) < 1e-7
+
+
+def test_bbox_sampling_properties():
+    o, d = orc.make_rays(50, seed=4)
+    n = 16
+    pts, ts, steps = orc.sample_points_along_ray_within_bbox(o, d, BOX, n, np.full(n, 0.5))
+    t0, t1 = orc.ray_segment_from_bounding_box(o, d, BOX)
+    # midpoints of n equal segments
+    want = t0[:, None] + (np.arange(n) + 0.5)[None] * ((t1 - t0) / n)[:, None]
+    np.testing.assert_allclose(ts, want, rtol=1e-13)
+    np.testing.assert_allclose(steps, np.broadcast_to(((t1 - t0) / n)[:, None], ts.shape))
+    np.testing.assert_allclose(pts, o.T[:, :, None] + d.T[:, :, None] * ts[None], rtol=1e-13)
+    assert np.all(np.diff(ts, axis=1) > 0)
+
+
+def test_contraction_properties():
+    o, d = orc.make_rays(64, seed=5, radius=3.0)
+    n = 32
+    jit = RNG.uniform(size=(64, n))
+    pts, ts, steps = orc.sample_contracted(o, d, n, 0.05, 200.0, jit)
+    assert np.all(np.abs(pts) <= 2.0 + 1e-12)
+    raw = o[:, None, :] + ts[:, :, None] * d[:, None, :]
+    nrm = np.abs(raw).max(-1)
+    cn = np.abs(np.moveaxis(pts, 0, -1)).max(-1)
+    inside = nrm <= 1.0
+    np.testing.assert_allclose(np.moveaxis(pts, 0, -1)[inside], raw[inside])
+    np.testing.assert_allclose(cn[~inside], 2.0 - 1.0 / nrm[~inside], rtol=1e-12)
+    base, st = orc.contraction_schedule(n, 0.05, 200.0)
+    assert base.dtype == np.float32 and np.all(np.diff(base) > 0)
+    assert abs(base[0] - 0.05) < 1e-7 and abs(base[n // 2 - 1] - 1.05) < 1e-6
+    # far end: far_start + (1/(1-(1-1/((far-fs)/k+1))) - 1) * k == far
+    assert abs(base[-1] - 200.0) < 1e-3
+    assert st[-1] == st[-2]
+
+
+def test_compositing_invariants_and_closed_form():
+    sigma = RNG.uniform(0, 30, size=(20, 40))
+    steps = RNG.uniform(0.001, 0.05, size=(20, 40))
+    e, w = orc.compute_segment_probabilities(sigma, steps)
+    np.testing.assert_allclose(w.sum(1) + e[:, -1], 1.0, atol=1e-12)   # telescoping
+    assert np.all(np.diff(e, axis=1) <= 0)
+    # brute force product of survival probabilities
+    surv = np.exp(-sigma * steps)
+    for r in range(3):
+        for i in range(40):
+            t = np.prod(surv[r, :i])
+            assert abs(w[r, i] - (1 - surv[r, i]) * t) < 1e-14
+            assert abs(e[r, i] - t * surv[r, i]) < 1e-14
+    e1, w1 = orc.compute_segment_probabilities(np.array([[2.0]]), np.array([[0.5]]))
+    assert abs(w1[0, 0] - (1 - math.exp(-1))) < 1e-15 and abs(e1[0, 0] - math.exp(-1)) < 1e-15
+
+
+def test_topk_brute_force():
+    w = RNG.uniform(size=(30, 24)).round(1)  # many ties
+    k = 7
+    idx = orc.topk_indices(w, k)
+    for r in range(30):
+        want = sorted(sorted(range(24), key=lambda i: (-w[r, i], i))[:k])
+        assert list(idx[r]) == want
+    # k == N selects everything
+    assert np.array_equal(orc.topk_indices(w, 24), np.tile(np.arange(24), (30, 1)))
+    # Gumbel form with zero noise == deterministic form for distinct weights
+    wd = RNG.uniform(0.01, 1, size=(10, 24))
+    assert np.array_equal(orc.topk_indices(wd, 5, np.zeros(24, np.float32)), orc.topk_indices(wd, 5))
+
+
+def test_fourier_closed_form():
+    x = RNG.normal(size=(5, 3))
+    fe = orc.fourier_encode(x, 3).reshape(5, 3, 6)
+    for q in range(3):
+        np.testing.assert_allclose(fe[:, :, q], np.sin(2 ** q * x), atol=1e-14)
+        np.testing.assert_allclose(fe[:, :, 3 + q], np.cos(2 ** q * x), atol=1e-14)
+
+
+@pytest.mark.parametrize("ncam", [None, 4])
+def test_mlp_forward_backward_vs_torch(ncam):
+    shape = orc.MlpShape(feat_dim=12, squash_dim=5, units=8, feature_n_freqs=2,
+                         viewdir_n_freqs=2, num_cameras=ncam)
+    p = orc.init_mlp_params(shape, RNG)
+    m = 33
+    f = RNG.normal(size=(m, 12))
+    d = RNG.normal(size=(m, 3))
+    cams = RNG.integers(0, 4, size=m)
+    rgb, cache = orc.feature_mlp_forward(p, f, d, cams, shape)
+    tp = torch_ref.to_torch(p)
+    tf = torch.tensor(f, requires_grad=True)
+    trgb = torch_ref.mlp_torch(tp, tf, torch.tensor(d), torch.tensor(cams), shape)
+    np.testing.assert_allclose(rgb, trgb.detach().numpy(), rtol=1e-12, atol=1e-13)
+    g = RNG.normal(size=(m, 3))
+    grads, gf = orc.feature_mlp_backward(p, cache, g, shape)
+    (trgb * torch.tensor(g)).sum().backward()
+    np.testing.assert_allclose(gf, tf.grad.numpy(), rtol=1e-10, atol=1e-12)
+    for k, v in grads.items():
+        np.testing.assert_allclose(v, tp[k].grad.numpy(), rtol=1e-10, atol=1e-12, err_msg=k)
+
+
+def _small_problem(k=6, n=20, ncam=None):
+    rng = np.random.default_rng(2024)
+    shape = orc.MlpShape(feat_dim=3 * 3, squash_dim=4, units=8, feature_n_freqs=2,
+                         viewdir_n_freqs=2, num_cameras=ncam)
+    p = orc.make_params(6, 2, 3, shape, seed=11, std=0.3)
+    for b in ("b1", "b2", "b3"):  # non-zero biases keep ReLU inputs away from the kink at 0
+        p[b] = rng.normal(0, 0.3, size=p[b].shape)
+    o, d = orc.make_rays(5, seed=12, radius=4.0, target_half=0.8)
+    cams = np.arange(5) % (ncam or 1)
+    cfg = orc.RenderConfig(density_samples_per_ray=n, appearance_samples_per_ray=k)
+    jit = rng.uniform(size=n)
+    return shape, p, o, d, cams, cfg, jit
+
+
+@pytest.mark.parametrize("k,ncam", [(6, None), (20, None), (6, 3)])
+def test_render_backward_vs_torch_autograd(k, ncam):
+    shape, p, o, d, cams, cfg, jit = _small_problem(k=k, ncam=ncam)
+    box = np.array([[-1.0] * 3, [1.0] * 3])
+    out, cache = orc.render_rays(p, box, o, d, cams, cfg, shape, jit)
+    tp = torch_ref.to_torch(p)
+    xhat = torch.tensor(cache["xhat"])
+    tout, tidx = torch_ref.render_rgb_torch(tp, xhat, torch.tensor(cache["steps"]),
+                                            torch.tensor(d), torch.tensor(cams), k, shape)
+    assert np.array_equal(tidx.numpy(), cache["idx"])
+    np.testing.assert_allclose(out, tout.detach().numpy(), rtol=1e-11, atol=1e-12)
+    g = RNG.normal(size=out.shape)
+    grads = orc.render_rays_backward(p, cache, g, cfg, shape)
+    (tout * torch.tensor(g)).sum().backward()
+    for key, v in grads.items():
+        np.testing.assert_allclose(v, tp[key].grad.numpy(), rtol=1e-8, atol=1e-11, err_msg=key)
+
+
+def test_render_backward_central_fd():
+    shape, p, o, d, cams, cfg, jit = _small_problem(k=6)
+    box = np.array([[-1.0] * 3, [1.0] * 3])
+    out, cache = orc.render_rays(p, box, o, d, cams, cfg, shape, jit)
+    g = RNG.normal(size=out.shape)
+    grads = orc.render_rays_backward(p, cache, g, cfg, shape)
+    idx0 = cache["idx"]
+
+    def loss(pp):
+        o2, c2 = orc.render_rays(pp, box, o, d, cams, cfg, shape, jit)
+        assert np.array_equal(c2["idx"], idx0)  # selection must not flip under the step
+        return float((o2 * g).sum())
+
+    h = 1e-6
+    for key in ("density_matrix", "density_vector", "appearance_matrix", "appearance_vector",
+                "w0", "w1", "b2", "w3"):
+        flat_g = grads[key].ravel()
+        cand = np.argsort(-np.abs(flat_g))[:3]
+        for j in cand:
+            pp, pm = {k: v.copy() for k, v in p.items()}, {k: v.copy() for k, v in p.items()}
+            pp[key].ravel()[j] += h
+            pm[key].ravel()[j] -= h
+            fd = (loss(pp) - loss(pm)) / (2 * h)
+            assert abs(fd - flat_g[j]) < 1e-6 + 1e-5 * abs(fd), (key, j, fd, flat_g[j])
+
+
+def test_depth_modes():
+    shape, p, o, d, cams, cfg, jit = _small_problem()
+    box = np.array([[-1.0] * 3, [1.0] * 3])
+    cfg.mode = orc.DIST_MEAN
+    mean, c = orc.render_rays(p, box, o, d, cams, cfg, shape, jit)
+    np.testing.assert_allclose(mean, (c["w"] * c["ts"]).sum(1) + c["p_exits"][:, -1] * c["ts"][:, -1])
+    cfg.mode = orc.DIST_MEDIAN
+    med, c = orc.render_rays(p, box, o, d, cams, cfg, shape, jit)
+    for r in range(len(med)):
+        alive = 1 - c["p_exits"][r]
+        above = np.nonzero(alive > 0.5)[0]
+        if len(above) == 0:
+            assert med[r] == np.inf
+        elif above[0] == 0:
+            assert med[r] == 0.0
+        else:
+            assert med[r] == c["ts"][r, above[0]]
+
+
+def test_adam_vs_torch_and_decay():
+    pshape = (40,)
+    p0 = RNG.normal(size=pshape)
+    torch_p = torch.tensor(p0, requires_grad=True)
+    lr, decay = 0.02, 0.7
+    opt = torch.optim.Adam([torch_p], lr=lr * decay, betas=(0.9, 0.99), eps=1e-8)
+    p, m, v = p0.copy(), np.zeros(pshape), np.zeros(pshape)
+    for count in range(1, 6):
+        g = RNG.normal(size=pshape)
+        torch_p.grad = torch.tensor(g)
+        opt.step()
+        p, m, v = orc.adam_update(p, g, m, v, count, lr, decay)
+        np.testing.assert_allclose(p, torch_p.detach().numpy(), rtol=1e-12, atol=1e-14)
+    oc = orc.OptimizerConfig()
+    ups = (2000, 3000, 4000, 5500, 7000)
+    assert orc.lr_decay_coeff(0, 30000, ups, oc) == 1.0
+    assert abs(orc.lr_decay_coeff(2000, 30000, ups, oc) - 1.0) < 1e-7   # reset at upsample
+    assert abs(orc.lr_decay_coeff(1999, 30000, ups, oc) - 0.1 ** (1999 / 30000)) < 1e-6
+    assert abs(orc.lr_decay_coeff(30000, 30000, (), oc) - 0.1) < 1e-7
+    assert abs(orc.lr_decay_coeff(60000, 30000, (), oc) - 0.1) < 1e-7    # floored at end_value
+
+
+def test_train_step_reduces_loss():
+    shape, p, o, d, cams, cfg, jit = _small_problem(k=20)
+    box = np.array([[-1.0] * 3, [1.0] * 3])
+    colors = np.random.default_rng(99).uniform(size=(5, 3))
+    st = orc.init_opt_state(p)
+    losses = []
+    for step in range(40):
+        p, st, _, logs, _, _ = orc.train_step(p, st, step, box, o, d, cams, colors, cfg, shape,
+                                              jit, n_iters=100)
+        losses.append(logs["mse"])
+    assert losses[-1] < 0.9 * losses[0]
