@@ -1,0 +1,243 @@
+"""Benchmark of the TensoRF training step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (the plain uniform-512 FP32 configuration the metric is quoted on):
+one TrainState.training_step (training.py:101-205) = render 4096 rays x 512
+stratified samples in the [-1.5,1.5]^3 box through a 300^3 VM grid (density
+16, appearance 48 channels per mode), every sample shaded by the FP32 feature
+MLP (144->27, 150->128->128->3, 2+2 Fourier frequencies), MSE loss,
+hand-written backward, Adam on all 17.3 M parameters. Rays: origins on the
+upper hemisphere of radius 4 aimed at uniform points of [-0.8,0.8]^3;
+targets uniform in [0,1] (synthetic data, resident in HBM before timing).
+
+Prints ONE JSON line with train rays/s (value), render rays/s, the live
+roofline of the dominant kernel (HIP events on the library's launch stream)
+and the CPU baseline (the oracle timed on host cores on a bounded sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "tensorf-jax_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "train rays/s (fwd+bwd) & render rays/s; proposal vs uniform at <=0.1 dB PSNR"
+
+# MI355X published peaks
+PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0}
+
+
+def stage_model(stage, R, N, k, cd, ca, precision, n_params):
+    """Algorithmic work of one launch of a stage: (bound, amount, unit).
+    Bytes: VM taps 72*C per sample (3 modes x 6 taps x C x 4 B) plus the
+    streamed per-sample I/O; FLOPs: 79,712 per MLP row forward (nf=nv=2)."""
+    M = R * k
+    mlp_row = 2 * (144 * 27 + 150 * 128 + 128 * 128 + 128 * 3)
+    if stage == "density_pass":
+        return "hbm", R * N * (72 * cd + 28), "GB/s"
+    if stage == "appearance_gather":
+        return "hbm", M * (72 * ca + 12 + 4 * 3 * ca), "GB/s"
+    if stage == "mlp_fwd_kernel":
+        return "mfma", M * mlp_row, "TFLOP/s"
+    if stage == "mlp_bwd":
+        return "mfma", M * 3 * mlp_row, "TFLOP/s"
+    if stage == "composite_bwd_density_scatter":
+        return "hbm", R * N * (2 * 72 * cd + 36), "GB/s"
+    if stage == "appearance_scatter":
+        return "hbm", M * (2 * 72 * ca + 12 + 4 * 3 * ca), "GB/s"
+    if stage == "adam":
+        return "hbm", n_params * 32, "GB/s"
+    return None
+
+
+def make_batches(R, count, seed, device):
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    out = []
+    for _ in range(count):
+        v = torch.randn((R, 3), generator=g, dtype=torch.float64)
+        v[:, 2] = v[:, 2].abs()
+        o = 4.0 * v / v.norm(dim=1, keepdim=True)
+        tgt = (torch.rand((R, 3), generator=g, dtype=torch.float64) * 2 - 1) * 0.8
+        d = tgt - o
+        d = d / d.norm(dim=1, keepdim=True)
+        colors = torch.rand((R, 3), generator=g)
+        out.append((o.float().to(device), d.float().to(device), colors.to(device)))
+    return out
+
+
+def cpu_baseline(R_sample, N, k, G, cd, ca):
+    """The oracle (NumPy FP32 restatement of the reference) timed for one
+    training step on R_sample rays of the same workload, single-threaded."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    from oracle import tensorf_oracle as orc
+    shape = orc.MlpShape()
+    p = orc.make_params(G, cd, ca, shape, seed=0)
+    p = {key: v.astype(np.float32) for key, v in p.items()}
+    o, d = orc.make_rays(R_sample, seed=1, radius=4.0, target_half=0.8)
+    colors = np.random.default_rng(2).uniform(size=(R_sample, 3))
+    cfg = orc.RenderConfig(0.05, 200.0, orc.RGB, N, k)
+    st = orc.init_opt_state(p)
+    jit = np.random.default_rng(3).uniform(size=N)
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        orc.train_step(p, st, 0, np.array([[-1.5] * 3, [1.5] * 3]), o, d, np.zeros(R_sample),
+                       colors, cfg, shape, jit, n_iters=30000, dtype=np.float32)
+        dt = time.perf_counter() - t0
+    return {"value": R_sample / dt, "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": f"one oracle training step (fwd + analytic bwd + Adam over all params) on "
+                      f"{R_sample} of the {N}-sample rays at G={G}, NumPy float32, 1 thread; "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--rays", type=int, default=4096)
+    ap.add_argument("--samples", type=int, default=512)
+    ap.add_argument("--appearance-samples", type=int, default=512)
+    ap.add_argument("--grid", type=int, default=300)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--cpu-sample-rays", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stages", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the step would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from tensorf_amd import _lib, cameras, render, training
+
+    torch.cuda.set_device(0)
+    R, N, k, G = args.rays, args.samples, args.appearance_samples, args.grid
+    cd, ca = 16, 48
+    cfg = training.TensorfConfig(
+        minibatch_size=R, n_iters=30000, initial_aabb_min=(-1.5,) * 3,
+        initial_aabb_max=(1.5,) * 3, appearance_feat_dim=ca, density_feat_dim=cd,
+        feature_n_freqs=2, viewdir_n_freqs=2, grid_dim_init=G, grid_dim_final=G, upsamp_iters=(),
+        density_samples_per_ray=N, appearance_samples_per_ray=k, mlp_precision=args.precision)
+    state = training.TrainState.initialize(cfg, G, prng_key=0, num_cameras=1)
+    batches = make_batches(R, 4, seed=1, device="cuda")
+    cams = torch.zeros(R, dtype=torch.int32, device="cuda")
+    mbs = [cameras.RenderedRays(c, cameras.Rays3D(o, d, cams)) for o, d, c in batches]
+
+    def step(i):
+        state.training_step(mbs[i % len(mbs)])
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(i)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = R * args.steps / dt
+    mse = float(state._mse.item())
+
+    # render rays/s (RGB forward, no gradient) on the same batches
+    lp = state.learnable_params
+    rcfg = state.render_config()
+    ws = state._ws
+    rays = [m.rays_wrt_world for m in mbs]
+
+    def render_once(i):
+        env = render._RenderEnv(state.appearance_mlp, lp, state.aabb, rays[i % len(rays)], 7,
+                                rcfg, ws)
+        env.forward()
+
+    for i in range(2):
+        render_once(i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    n_render = max(3, args.steps // 2)
+    for i in range(n_render):
+        render_once(i)
+    torch.cuda.synchronize()
+    render_value = R * n_render / (time.perf_counter() - t1)
+
+    # live per-stage timing (HIP events on the launch stream) of a few steps
+    stages = {}
+    roofline = None
+    if not args.no_stages:
+        _lib.profile_enable(True)
+        n_prof = 3
+        for i in range(n_prof):
+            step(i)
+        torch.cuda.synchronize()
+        rec = _lib.profile_read()
+        _lib.profile_enable(False)
+        n_params = state.flat.total
+        best = None
+        for name, (ms, cnt) in rec.items():
+            avg = ms / max(cnt, 1)
+            stages[name] = round(avg, 4)
+            model = stage_model(name, R, N, k, cd, ca, args.precision, n_params)
+            if model is not None and (best is None or avg > best[1]):
+                best = (name, avg, model)
+        if best is not None:
+            name, avg, (bound, amount, unit) = best
+            if unit == "GB/s":
+                achieved = amount / (avg * 1e-3) / 1e9
+                peak = PEAK_HBM_GBS
+            else:
+                achieved = amount / (avg * 1e-3) / 1e12
+                peak = PEAK_TFLOPS[args.precision]
+            traffic = None
+            tpath = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+            if os.path.exists(tpath):
+                with open(tpath) as f:
+                    tr = json.load(f)
+                if tr.get("workload") == f"R{R}_N{N}_k{k}_G{G}_{args.precision}":
+                    traffic = tr.get("bytes_per_launch")
+            roofline = {"kernel": name, "bound": bound, "achieved": round(achieved, 3),
+                        "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
+                        "traffic": traffic, "avg_ms": round(avg, 4),
+                        "algorithmic_per_launch": amount}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_sample_rays, N, k, G, cd, ca)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"plain TensoRF training step: R={R} rays x N={N} stratified "
+                               f"samples, k={k} shaded, VM {G}^3 (Cd={cd}, Ca={ca}), "
+                               f"{args.precision} MLP, Adam",
+                   "rays_per_step": R, "samples_per_ray": N, "appearance_samples_per_ray": k,
+                   "grid": G, "parallelism": "single GPU"},
+        "render_rays_per_s": round(render_value, 1),
+        "final_mse": mse,
+        "stages_ms": stages,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

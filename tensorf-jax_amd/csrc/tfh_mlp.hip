@@ -1,0 +1,738 @@
+// FeatureMlp (networks.py:38-121) forward and backward on gfx950 MFMA.
+//
+// Layers (kernels are (in, out) like Flax Dense):
+//   h0 = f @ W0                       (F -> S, no bias)      networks.py:57-61
+//   x  = [h0, d, FE(h0, nf), FE(d, nv)]  (E wide)            networks.py:68-76
+//   h1 = relu(x @ W1 + b1)            (E -> U)               networks.py:86-90
+//   h2 = relu(h1 @ W2 + b2)           (U -> U)               networks.py:94-98
+//   h2[U/2:] = e[:U/2] * h2[U/2:] + e[U/2:],  e = embed[cam]  networks.py:103-111
+//   rgb = sigmoid(h2 @ W3 + b3)       (U -> 3)               networks.py:114-120
+//
+// A workgroup owns a tile of TM rows. Activations live in LDS (row-major,
+// padded strides that keep MFMA operand reads bank-conflict free); every
+// layer is a set of 16x16 output tiles computed with
+// v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f32_16x16x32_bf16 (BF16 operands,
+// FP32 accumulation). Weights are read from global memory (L2-resident).
+//
+// The backward recomputes the forward of its tile and keeps weight gradients
+// in a per-workgroup partial buffer (plain read-modify-write, each partial
+// tile owned by one wave), reduced across workgroups by k_mlp_reduce.
+#include "tfh_common.h"
+
+namespace tfh {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kU = 128;   // hidden width
+constexpr int kSP = 32;   // padded squash width
+
+struct MlpDims {
+  int F, FP;   // feature dim, padded to 16
+  int S;       // squash dim (<= 32)
+  int E, EP;   // encoded dim, padded to 16
+  int nf, nv;
+  int ncam;
+  // LDS strides (floats)
+  int ldF, ldE, ldU, ldS, ld3;
+};
+
+__host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+inline MlpDims make_dims(const tfh_mlp* p) {
+  MlpDims d;
+  d.F = (int)p->feat_dim;
+  d.FP = round_up(d.F, 16);
+  d.S = (int)p->squash_dim;
+  d.nf = (int)p->feature_n_freqs;
+  d.nv = (int)p->viewdir_n_freqs;
+  d.E = d.S + 3 + 2 * d.nf * d.S + 2 * d.nv * 3;
+  d.EP = round_up(d.E, 16);
+  d.ncam = p->embed ? (int)p->num_cameras : 0;
+  d.ldF = d.FP + 2;
+  d.ldE = d.EP + 2;
+  d.ldU = kU + 2;
+  d.ldS = kSP + 2;
+  d.ld3 = 16 + 2;
+  return d;
+}
+
+struct MlpW {  // device pointers of the parameters
+  const float *w0, *w1, *b1, *w2, *b2, *w3, *b3, *embed;
+};
+
+// ---- MFMA tile helpers --------------------------------------------------------
+// D layout of a 16x16 tile: lane l, element r -> row 4*(l/16)+r, col l%16.
+// A operand: lane l holds A[l%16][k0 + l/16]; B operand: B[k0 + l/16][l%16].
+
+template <int NR>
+__device__ __forceinline__ void zero_acc(f32x4 (&acc)[NR]) {
+#pragma unroll
+  for (int i = 0; i < NR; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+}
+
+// acc[rt] += A(rows 16*rt.., K) * B(K, 16 cols), rt < NR; A from LDS
+// (row-major, lda), B(k, j) = bsrc(k, j).
+template <int NR, class BSrc>
+__device__ __forceinline__ void mma_strip_f32(const float* __restrict__ A, int lda, int K,
+                                              BSrc bsrc, int col0, f32x4 (&acc)[NR]) {
+  const int lane = lane_id();
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll 4
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const float b = bsrc(k0 + kq, col0 + r);
+#pragma unroll
+    for (int rt = 0; rt < NR; ++rt) {
+      const float a = A[(16 * rt + r) * lda + k0 + kq];
+      acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[rt], 0, 0, 0);
+    }
+  }
+}
+
+// acc += A^T-style product for weight gradients:
+// acc(i, j) += sum_k X[k][row0 + i] * Y[k][col0 + j], k < K (rows of the tile)
+__device__ __forceinline__ f32x4 mma_wgrad_f32(const float* __restrict__ X, int ldx, int row0,
+                                               const float* __restrict__ Y, int ldy, int col0,
+                                               int K) {
+  const int lane = lane_id();
+  const int r = lane & 15, kq = lane >> 4;
+  f32x4 acc{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const float a = X[(k0 + kq) * ldx + row0 + r];
+    const float b = Y[(k0 + kq) * ldy + col0 + r];
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// Row-major global matrix as a B operand with zero padding outside (rows, cols).
+struct GlobalB {
+  const float* w;
+  int rows, cols;
+  __device__ __forceinline__ float operator()(int k, int j) const {
+    return (k < rows && j < cols) ? w[k * cols + j] : 0.0f;
+  }
+};
+
+// ---- shared forward pieces ------------------------------------------------------
+// Loads TM rows of features (zero padded to FP columns and past M), view
+// directions and camera indices of the tile.
+template <int TM>
+__device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
+                          const float* __restrict__ dirs, const uint32_t* __restrict__ cams,
+                          int64_t row0, int64_t M, int64_t rpr, float* Fs, float* Ds,
+                          int* Cs) {
+  for (int e = threadIdx.x; e < TM * d.FP; e += blockDim.x) {
+    const int i = e / d.FP, c = e % d.FP;
+    const int64_t row = row0 + i;
+    Fs[i * d.ldF + c] = (row < M && c < d.F) ? feats[row * d.F + c] : 0.0f;
+  }
+  for (int e = threadIdx.x; e < TM * 3; e += blockDim.x) {
+    const int i = e / 3, c = e % 3;
+    const int64_t row = row0 + i;
+    Ds[i * 3 + c] = row < M ? dirs[(row / rpr) * 3 + c] : 0.0f;
+  }
+  for (int i = threadIdx.x; i < TM; i += blockDim.x) {
+    const int64_t row = row0 + i;
+    int cam = (cams && row < M) ? (int)cams[row / rpr] : 0;
+    if (cam < 0 || cam >= d.ncam) cam = 0;
+    Cs[i] = cam;
+  }
+}
+
+// Fills X[:, S:EP] = [d, FE(h0, nf), FE(d, nv), 0...] from h0 = X[:, 0:S]
+// (networks.py:13-35, 68-76): FE(v, n)[j*2n + q] = sin(2^q v_j) for q < n and
+// sin(2^(q-n) v_j + pi/2) otherwise.
+template <int TM>
+__device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
+  const int ncol = d.EP - d.S;
+  const int fe0 = d.S + 3;
+  const int fe1 = fe0 + 2 * d.nf * d.S;
+  const int fe2 = fe1 + 2 * d.nv * 3;
+  for (int e = threadIdx.x; e < TM * ncol; e += blockDim.x) {
+    const int i = e / ncol, c = d.S + e % ncol;
+    float v;
+    if (c < fe0) {
+      v = Ds[i * 3 + (c - d.S)];
+    } else if (c < fe2) {
+      const bool feat = c < fe1;
+      const int t = feat ? c - fe0 : c - fe1;
+      const int n = feat ? d.nf : d.nv;
+      const int j = t / (2 * n), q = t % (2 * n);
+      const float src = feat ? X[i * d.ldE + j] : Ds[i * 3 + j];
+      const float scale = (float)(1 << (q < n ? q : q - n));
+      const float arg = src * scale;
+      v = q < n ? sinf(arg) : sinf(arg + 1.57079632679489662f);
+    } else {
+      v = 0.0f;
+    }
+    X[i * d.ldE + c] = v;
+  }
+}
+
+// Runs layers 0..2 of the tile. On exit: X holds the encoded input, H0 (if
+// non-null) a copy of h0, H1 = relu(z1), H2p = relu(z2) (pre-FiLM) and H2 the
+// layer-3 input (== H2p without embedding). Fs is read by layer 0 only.
+template <int TM, int NW>
+__device__ void forward_hidden(const MlpDims& d, const MlpW& w, const float* Fs, float* X,
+                               float* H0, float* H1, float* H2p, float* H2, const float* Ds,
+                               const int* Cs) {
+  constexpr int RT = TM / 16;
+  const int wave = threadIdx.x >> 6;
+  const int lane = lane_id();
+  // layer 0: (TM x F) @ (F x S) -> X[:, 0:S] (2 column tiles of 16)
+  for (int ct = wave; ct < 2; ct += NW) {
+    f32x4 acc[RT];
+    zero_acc(acc);
+    mma_strip_f32<RT>(Fs, d.ldF, round_up(d.F, 4), GlobalB{w.w0, d.F, d.S}, 16 * ct, acc);
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 16 * rt + 4 * (lane >> 4) + e, col = 16 * ct + (lane & 15);
+        if (col < d.S) {
+          X[row * d.ldE + col] = acc[rt][e];
+          if (H0) H0[row * d.ldS + col] = acc[rt][e];
+        }
+      }
+  }
+  __syncthreads();
+  encode_tile<TM>(d, X, Ds);
+  __syncthreads();
+  // layer 1: (TM x EP) @ (EP x U) + b1, relu -> H1
+  for (int ct = wave; ct < kU / 16; ct += NW) {
+    f32x4 acc[RT];
+    zero_acc(acc);
+    mma_strip_f32<RT>(X, d.ldE, d.EP, GlobalB{w.w1, d.E, kU}, 16 * ct, acc);
+    const int col = 16 * ct + (lane & 15);
+    const float bias = w.b1[col];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 16 * rt + 4 * (lane >> 4) + e;
+        H1[row * d.ldU + col] = fmaxf(acc[rt][e] + bias, 0.0f);
+      }
+  }
+  __syncthreads();
+  // layer 2: (TM x U) @ (U x U) + b2, relu (+ FiLM) -> H2p / H2
+  for (int ct = wave; ct < kU / 16; ct += NW) {
+    f32x4 acc[RT];
+    zero_acc(acc);
+    mma_strip_f32<RT>(H1, d.ldU, kU, GlobalB{w.w2, kU, kU}, 16 * ct, acc);
+    const int col = 16 * ct + (lane & 15);
+    const float bias = w.b2[col];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 16 * rt + 4 * (lane >> 4) + e;
+        const float h = fmaxf(acc[rt][e] + bias, 0.0f);
+        float out = h;
+        if (d.ncam > 0 && col >= kU / 2) {
+          const float* emb = w.embed + (size_t)Cs[row] * kU;
+          out = emb[col - kU / 2] * h + emb[col];
+        }
+        if (H2p != H2) H2p[row * d.ldU + col] = h;
+        H2[row * d.ldU + col] = out;
+      }
+  }
+  __syncthreads();
+}
+
+// layer 3 + sigmoid: rgb[row][c] for c < 3, rows of the tile. Writes
+// sigmoid values into R3 (TM x ld3, columns >= 3 untouched).
+template <int TM, int NW>
+__device__ void forward_out(const MlpDims& d, const MlpW& w, const float* H2, float* R3) {
+  constexpr int RT = TM / 16;
+  const int wave = threadIdx.x >> 6;
+  const int lane = lane_id();
+  // one 16-wide column tile; distribute row tiles over waves
+  for (int rt = wave; rt < RT; rt += NW) {
+    f32x4 acc[1];
+    zero_acc(acc);
+    mma_strip_f32<1>(H2 + 16 * rt * d.ldU, d.ldU, kU, GlobalB{w.w3, kU, 3}, 0, acc);
+    const int col = lane & 15;
+    if (col < 3) {
+      const float bias = w.b3[col];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 16 * rt + 4 * (lane >> 4) + e;
+        const float z = acc[0][e] + bias;
+        R3[row * d.ld3 + col] = 1.0f / (1.0f + expf(-z));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---- forward kernel -----------------------------------------------------------
+template <int TM, int NW>
+__global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d, MlpW w,
+                                                     const float* __restrict__ feats,
+                                                     const float* __restrict__ dirs,
+                                                     const uint32_t* __restrict__ cams,
+                                                     int64_t M, int64_t rpr,
+                                                     float* __restrict__ rgb) {
+  extern __shared__ float smem[];
+  // regions: A = max(Fs, H1), B = max(X, H2), small: Ds, Cs, R3
+  float* regA = smem;
+  float* regB = regA + TM * (d.ldF > d.ldU ? d.ldF : d.ldU);
+  float* Ds = regB + TM * (d.ldE > d.ldU ? d.ldE : d.ldU);
+  float* R3 = Ds + TM * 3;
+  int* Cs = reinterpret_cast<int*>(R3 + TM * d.ld3);
+  const int64_t row0 = (int64_t)blockIdx.x * TM;
+  load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, regA, Ds, Cs);
+  __syncthreads();
+  // Fs = regA, X = regB, H1 = regA (after layer 0), H2 = regB (after layer 1)
+  forward_hidden<TM, NW>(d, w, regA, regB, nullptr, regA, regB, regB, Ds, Cs);
+  forward_out<TM, NW>(d, w, regB, R3);
+  for (int e = threadIdx.x; e < TM * 3; e += blockDim.x) {
+    const int i = e / 3, c = e % 3;
+    const int64_t row = row0 + i;
+    if (row < M) rgb[row * 3 + c] = R3[i * d.ld3 + c];
+  }
+}
+
+// NOTE: forward_hidden reads Fs in layer 0 and writes H1 in layer 1 into the
+// same region; layer 0 completes (barrier) before layer 1 starts. X and H2
+// share a region: layer 2 reads H1 and writes H2 after layer 1 consumed X.
+
+template <int TM>
+inline size_t fwd_lds_bytes(const MlpDims& d) {
+  size_t a = (size_t)TM * (d.ldF > d.ldU ? d.ldF : d.ldU);
+  size_t b = (size_t)TM * (d.ldE > d.ldU ? d.ldE : d.ldU);
+  return (a + b + TM * 3 + TM * d.ld3 + TM) * sizeof(float);
+}
+
+// ---- backward ---------------------------------------------------------------------
+// Packed layout of one workgroup's gradient partial (and of the reduction).
+struct GradPack {
+  int64_t w0, w1, b1, w2, b2, w3, b3, total;
+};
+
+inline GradPack make_pack(const MlpDims& d) {
+  GradPack p;
+  int64_t o = 0;
+  p.w0 = o; o += (int64_t)d.F * d.S;
+  p.w1 = o; o += (int64_t)d.E * kU;
+  p.b1 = o; o += kU;
+  p.w2 = o; o += (int64_t)kU * kU;
+  p.b2 = o; o += kU;
+  p.w3 = o; o += (int64_t)kU * 3;
+  p.b3 = o; o += 3;
+  p.total = round_up((int)o, 64);
+  return p;
+}
+
+// Transposed copies of the weights (B operands of the input-gradient GEMMs).
+struct MlpWT {
+  float *w0t, *w1t, *w2t, *w3t;  // (S,F), (U,E), (U,U), (3,U)
+};
+
+__global__ void k_transpose_weights(MlpDims d, MlpW w, MlpWT t) {
+  const int64_t n0 = (int64_t)d.F * d.S, n1 = (int64_t)d.E * kU, n2 = (int64_t)kU * kU,
+                n3 = (int64_t)kU * 3;
+  const int64_t total = n0 + n1 + n2 + n3;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (e < n0) {  // w0 (F, S) -> w0t (S, F)
+      const int64_t i = e / d.S, j = e % d.S;
+      t.w0t[j * d.F + i] = w.w0[e];
+    } else if (e < n0 + n1) {
+      const int64_t f = e - n0, i = f / kU, j = f % kU;
+      t.w1t[j * d.E + i] = w.w1[f];
+    } else if (e < n0 + n1 + n2) {
+      const int64_t f = e - n0 - n1, i = f / kU, j = f % kU;
+      t.w2t[j * kU + i] = w.w2[f];
+    } else {
+      const int64_t f = e - n0 - n1 - n2, i = f / 3, j = f % 3;
+      t.w3t[j * kU + i] = w.w3[f];
+    }
+  }
+}
+
+// Adds a 16x16 weight-gradient tile (D layout) into the workgroup partial at
+// rows [row0, row0+16) x cols [col0, col0+16) of a (rows x cols) block.
+__device__ __forceinline__ void add_tile(float* __restrict__ part, int64_t off, int rows,
+                                         int cols, int row0, int col0, const f32x4& acc) {
+  const int lane = lane_id();
+  const int col = col0 + (lane & 15);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int row = row0 + 4 * (lane >> 4) + e;
+    if (row < rows && col < cols) part[off + (int64_t)row * cols + col] += acc[e];
+  }
+}
+
+template <int TM, int NW>
+__global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt, GradPack gp,
+                                                     const float* __restrict__ feats,
+                                                     const float* __restrict__ dirs,
+                                                     const uint32_t* __restrict__ cams,
+                                                     const float* __restrict__ grgb, int64_t M,
+                                                     int64_t rpr, float* __restrict__ gfeat,
+                                                     float* __restrict__ partials,
+                                                     float* __restrict__ gembed) {
+  constexpr int RT = TM / 16;
+  extern __shared__ float smem[];
+  float* Fs = smem;                         // TM x ldF
+  float* X = Fs + TM * d.ldF;               // TM x ldE   (later dX)
+  float* H0 = X + TM * d.ldE;               // TM x ldS   (later dH0)
+  float* H1 = H0 + TM * d.ldS;              // TM x ldU   (later dZ1)
+  float* H2 = H1 + TM * d.ldU;              // TM x ldU   layer-3 input (later dZ2)
+  float* after_h2 = H2 + TM * d.ldU;
+  float* H2p = d.ncam > 0 ? after_h2 : H2;  // pre-FiLM relu(z2) (own region with FiLM)
+  float* Z3 = d.ncam > 0 ? after_h2 + TM * d.ldU : after_h2;  // TM x ld3: rgb, then dZ3
+  float* Ds = Z3 + TM * d.ld3;
+  int* Cs = reinterpret_cast<int*>(Ds + TM * 3);
+
+  const int wave = threadIdx.x >> 6;
+  const int lane = lane_id();
+  float* part = partials + (int64_t)blockIdx.x * gp.total;
+  for (int64_t e = threadIdx.x; e < gp.total; e += blockDim.x) part[e] = 0.0f;
+
+  const int64_t ntiles = (M + TM - 1) / TM;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t row0 = tile * TM;
+    __syncthreads();
+    load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
+    __syncthreads();
+    forward_hidden<TM, NW>(d, w, Fs, X, H0, H1, H2p, H2, Ds, Cs);
+    forward_out<TM, NW>(d, w, H2, Z3);
+    // dZ3 = grgb * s * (1 - s), zero outside the 3 colours and the valid rows
+    for (int e = threadIdx.x; e < TM * 16; e += blockDim.x) {
+      const int i = e / 16, c = e % 16;
+      const int64_t row = row0 + i;
+      float v = 0.0f;
+      if (c < 3 && row < M) {
+        const float s = Z3[i * d.ld3 + c];
+        v = grgb[row * 3 + c] * s * (1.0f - s);
+      }
+      Z3[i * d.ld3 + c] = v;
+    }
+    __syncthreads();
+    // dW3 (U x 3) += H2^T dZ3 ; db3
+    for (int t = wave; t < kU / 16; t += NW)
+      add_tile(part, gp.w3, kU, 3, 16 * t, 0, mma_wgrad_f32(H2, d.ldU, 16 * t, Z3, d.ld3, 0, TM));
+    if (threadIdx.x < 3) {
+      float s = 0.f;
+      for (int i = 0; i < TM; ++i) s += Z3[i * d.ld3 + threadIdx.x];
+      part[gp.b3 + threadIdx.x] += s;
+    }
+    __syncthreads();
+    // dH2 = dZ3 @ W3^T (K = 3 padded to 4); FiLM backward; dZ2 = dH2 * (h2p > 0)
+    for (int ct = wave; ct < kU / 16; ct += NW) {
+      f32x4 acc[RT];
+      zero_acc(acc);
+      mma_strip_f32<RT>(Z3, d.ld3, 4, GlobalB{wt.w3t, 3, kU}, 16 * ct, acc);
+      const int col = 16 * ct + (lane & 15);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = 16 * rt + 4 * (lane >> 4) + e;
+          float g = acc[rt][e];
+          const float hp = H2p[row * d.ldU + col];
+          if (d.ncam > 0 && col >= kU / 2 && row0 + row < M) {
+            const float* emb = w.embed + (size_t)Cs[row] * kU;
+            float* gemb = gembed + (size_t)Cs[row] * kU;
+            unsafeAtomicAdd(gemb + col - kU / 2, g * hp);
+            unsafeAtomicAdd(gemb + col, g);
+            g = g * emb[col - kU / 2];
+          }
+          H2[row * d.ldU + col] = hp > 0.0f ? g : 0.0f;  // dZ2 (H2 is dead after dW3)
+        }
+    }
+    __syncthreads();
+    // dW2 (U x U) += H1^T dZ2 ; db2
+    for (int t = wave; t < (kU / 16) * (kU / 16); t += NW) {
+      const int ti = t / (kU / 16), tj = t % (kU / 16);
+      add_tile(part, gp.w2, kU, kU, 16 * ti, 16 * tj,
+               mma_wgrad_f32(H1, d.ldU, 16 * ti, H2, d.ldU, 16 * tj, TM));
+    }
+    for (int c = threadIdx.x; c < kU; c += blockDim.x) {
+      float s = 0.f;
+      for (int i = 0; i < TM; ++i) s += H2[i * d.ldU + c];
+      part[gp.b2 + c] += s;
+    }
+    __syncthreads();
+    // dH1 = dZ2 @ W2^T ; dZ1 = dH1 * (h1 > 0) written over H1 (dW2 is done)
+    {
+      constexpr int CT = (kU / 16 + NW - 1) / NW;
+#pragma unroll
+      for (int s = 0; s < CT; ++s) {
+        const int ct = wave + s * NW;
+        if (ct < kU / 16) {
+          f32x4 acc[RT];
+          zero_acc(acc);
+          mma_strip_f32<RT>(H2, d.ldU, kU, GlobalB{wt.w2t, kU, kU}, 16 * ct, acc);
+          const int col = 16 * ct + (lane & 15);
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int row = 16 * rt + 4 * (lane >> 4) + e;
+              const float h = H1[row * d.ldU + col];
+              H1[row * d.ldU + col] = h > 0.0f ? acc[rt][e] : 0.0f;
+            }
+        }
+      }
+    }
+    __syncthreads();
+    // dW1 (E x U) += X^T dZ1 ; db1
+    for (int t = wave; t < (d.EP / 16) * (kU / 16); t += NW) {
+      const int ti = t / (kU / 16), tj = t % (kU / 16);
+      add_tile(part, gp.w1, d.E, kU, 16 * ti, 16 * tj,
+               mma_wgrad_f32(X, d.ldE, 16 * ti, H1, d.ldU, 16 * tj, TM));
+    }
+    for (int c = threadIdx.x; c < kU; c += blockDim.x) {
+      float s = 0.f;
+      for (int i = 0; i < TM; ++i) s += H1[i * d.ldU + c];
+      part[gp.b1 + c] += s;
+    }
+    __syncthreads();
+    // dX = dZ1 @ W1^T for the columns that feed h0: [0, S) and the FE(h0) block
+    {
+      const int fe0 = d.S + 3, fe1 = fe0 + 2 * d.nf * d.S;
+      const int nct = d.EP / 16;
+      for (int ct = wave; ct < nct; ct += NW) {
+        const int c0 = 16 * ct;
+        const bool needed = (c0 < d.S) || (c0 + 16 > fe0 && c0 < fe1);
+        if (!needed) continue;
+        f32x4 acc[RT];
+        zero_acc(acc);
+        mma_strip_f32<RT>(H1, d.ldU, kU, GlobalB{wt.w1t, kU, d.E}, c0, acc);
+        const int col = c0 + (lane & 15);
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = 16 * rt + 4 * (lane >> 4) + e;
+            X[row * d.ldE + col] = acc[rt][e];  // dX (X is dead after dW1)
+          }
+      }
+    }
+    __syncthreads();
+    // dH0[j] = dX[j] + sum_q dFE terms (networks.py:13-35); stored in H0 (zero padded)
+    {
+      const int fe0 = d.S + 3;
+      for (int e = threadIdx.x; e < TM * kSP; e += blockDim.x) {
+        const int i = e / kSP, j = e % kSP;
+        float g = 0.0f;
+        if (j < d.S) {
+          const float h = H0[i * d.ldS + j];
+          g = X[i * d.ldE + j];
+          for (int q = 0; q < d.nf; ++q) {
+            const float s = (float)(1 << q);
+            const float a = h * s;
+            g += X[i * d.ldE + fe0 + j * 2 * d.nf + q] * s * cosf(a);
+            g += X[i * d.ldE + fe0 + j * 2 * d.nf + d.nf + q] * s *
+                 cosf(a + 1.57079632679489662f);
+          }
+        }
+        H0[i * d.ldS + j] = g;
+      }
+    }
+    __syncthreads();
+    // dW0 (F x S) += F^T dH0
+    for (int t = wave; t < (d.FP / 16) * 2; t += NW) {
+      const int ti = t / 2, tj = t % 2;
+      add_tile(part, gp.w0, d.F, d.S, 16 * ti, 16 * tj,
+               mma_wgrad_f32(Fs, d.ldF, 16 * ti, H0, d.ldS, 16 * tj, TM));
+    }
+    // dF = dH0 @ W0^T -> global
+    if (gfeat) {
+      for (int ct = wave; ct < d.FP / 16; ct += NW) {
+        f32x4 acc[RT];
+        zero_acc(acc);
+        mma_strip_f32<RT>(H0, d.ldS, kSP, GlobalB{wt.w0t, d.S, d.F}, 16 * ct, acc);
+        const int col = 16 * ct + (lane & 15);
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = 16 * rt + 4 * (lane >> 4) + e;
+            const int64_t grow = row0 + row;
+            if (grow < M && col < d.F) gfeat[grow * d.F + col] = acc[rt][e];
+          }
+      }
+    }
+  }
+}
+
+template <int TM>
+inline size_t bwd_lds_bytes(const MlpDims& d) {
+  size_t f = (size_t)TM * (d.ldF + d.ldE + d.ldS + 2 * d.ldU + (d.ncam > 0 ? d.ldU : 0) + d.ld3 +
+                           3 + 1);
+  return f * sizeof(float);
+}
+
+// grads[e] += sum_b partials[b][e], mapped onto the separate gradient arrays.
+__global__ void k_mlp_reduce(const float* __restrict__ partials, int64_t nblk, GradPack gp,
+                             tfh_mlp_grad g) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < gp.total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (int64_t b = 0; b < nblk; ++b) s += partials[b * gp.total + e];
+    float* dst = nullptr;
+    int64_t o = 0;
+    if (e >= gp.b3 + 3) continue;
+    if (e >= gp.b3) { dst = g.b3; o = e - gp.b3; }
+    else if (e >= gp.w3) { dst = g.w3; o = e - gp.w3; }
+    else if (e >= gp.b2) { dst = g.b2; o = e - gp.b2; }
+    else if (e >= gp.w2) { dst = g.w2; o = e - gp.w2; }
+    else if (e >= gp.b1) { dst = g.b1; o = e - gp.b1; }
+    else if (e >= gp.w1) { dst = g.w1; o = e - gp.w1; }
+    else { dst = g.w0; o = e - gp.w0; }
+    dst[o] += s;
+  }
+}
+
+static int g_num_cus = 0;
+
+static int num_cus() {
+  if (g_num_cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      g_num_cus = n;
+    else
+      g_num_cus = 256;
+  }
+  return g_num_cus;
+}
+
+constexpr int kFwdTM = 32, kFwdNW = 4;
+constexpr int kBwdTM = 16, kBwdNW = 8;
+
+int32_t check_mlp(const tfh_mlp* p, const char* who) {
+  TFH_REQUIRE(p != nullptr, "%s: null tfh_mlp", who);
+  TFH_REQUIRE(p->w0 && p->w1 && p->b1 && p->w2 && p->b2 && p->w3 && p->b3,
+              "%s: null MLP parameter pointer", who);
+  TFH_REQUIRE(p->units == kU, "%s: units=%lld unsupported (kernels are built for %d)", who,
+              (long long)p->units, kU);
+  TFH_REQUIRE(p->squash_dim >= 1 && p->squash_dim <= kSP, "%s: squash_dim %lld outside [1, %d]",
+              who, (long long)p->squash_dim, kSP);
+  TFH_REQUIRE(p->feat_dim >= 1 && p->feat_dim <= 192, "%s: feat_dim %lld outside [1, 192]", who,
+              (long long)p->feat_dim);
+  TFH_REQUIRE(p->feature_n_freqs >= 0 && p->feature_n_freqs <= 8 && p->viewdir_n_freqs >= 0 &&
+                  p->viewdir_n_freqs <= 8,
+              "%s: Fourier frequency counts must be in [0, 8]", who);
+  TFH_REQUIRE(p->embed == nullptr || p->num_cameras >= 1, "%s: embed given with num_cameras=%lld",
+              who, (long long)p->num_cameras);
+  MlpDims d = make_dims(p);
+  TFH_REQUIRE(d.EP <= 448, "%s: encoded width %d too large", who, d.E);
+  return TFH_OK;
+}
+
+static MlpW to_w(const tfh_mlp* p) {
+  return MlpW{p->w0, p->w1, p->b1, p->w2, p->b2, p->w3, p->b3, p->embed};
+}
+
+int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
+                     int64_t M, int64_t rpr, float* rgb, int32_t precision, hipStream_t st) {
+  (void)precision;
+  MlpDims d = make_dims(p);
+  const size_t lds = fwd_lds_bytes<kFwdTM>(d);
+  TFH_REQUIRE(lds <= 160 * 1024, "mlp fwd: tile needs %zu bytes of LDS", lds);
+  const int64_t blocks = ceil_div(M, kFwdTM);
+  ProfScope ps("mlp_fwd_kernel", st);
+  hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW>), dim3((unsigned)blocks), dim3(kFwdNW * 64), lds,
+                     st, d, to_w(p), feats, dirs, cams, M, rpr, rgb);
+  return check_launch("mlp fwd");
+}
+
+size_t mlp_bwd_workspace(const tfh_mlp* p, int64_t M) {
+  MlpDims d = make_dims(p);
+  GradPack gp = make_pack(d);
+  int64_t nblk = std::min<int64_t>(ceil_div(M, kBwdTM), (int64_t)num_cus());
+  if (nblk < 1) nblk = 1;
+  size_t wt = (size_t)(d.F * d.S + d.E * kU + kU * kU + kU * 3);
+  return align_up(wt * sizeof(float), 256) + align_up((size_t)nblk * gp.total * sizeof(float), 256);
+}
+
+int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
+                     int64_t M, int64_t rpr, const float* grgb, float* gfeat, const tfh_mlp_grad* g,
+                     int32_t precision, void* ws, size_t ws_bytes, hipStream_t st) {
+  (void)precision;
+  MlpDims d = make_dims(p);
+  GradPack gp = make_pack(d);
+  int64_t nblk = std::min<int64_t>(ceil_div(M, kBwdTM), (int64_t)num_cus());
+  if (nblk < 1) nblk = 1;
+  Carve cv(ws, ws_bytes);
+  MlpWT wt;
+  wt.w0t = cv.take<float>((int64_t)d.F * d.S + (int64_t)d.E * kU + kU * kU + kU * 3);
+  float* partials = cv.take<float>(nblk * gp.total);
+  if (!cv.ok)
+    return fail(TFH_ERR_WORKSPACE, "mlp bwd: workspace of %zu bytes, need %zu", ws_bytes,
+                mlp_bwd_workspace(p, M));
+  wt.w1t = wt.w0t + (int64_t)d.F * d.S;
+  wt.w2t = wt.w1t + (int64_t)d.E * kU;
+  wt.w3t = wt.w2t + kU * kU;
+  hipLaunchKernelGGL(k_transpose_weights, dim3(64), dim3(256), 0, st, d, to_w(p), wt);
+  TFH_TRY(check_launch("mlp bwd: weight transpose"));
+  const size_t lds = bwd_lds_bytes<kBwdTM>(d);
+  TFH_REQUIRE(lds <= 160 * 1024, "mlp bwd: tile needs %zu bytes of LDS", lds);
+  {
+    ProfScope ps("mlp_bwd", st);
+    hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW>), dim3((unsigned)nblk), dim3(kBwdNW * 64), lds,
+                       st, d, to_w(p), wt, gp, feats, dirs, cams, grgb, M, rpr, gfeat, partials,
+                       g->embed);
+  }
+  TFH_TRY(check_launch("mlp bwd"));
+  ProfScope ps("mlp_reduce", st);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)ceil_div(gp.total, 256)), dim3(256), 0, st,
+                     partials, nblk, gp, *g);
+  return check_launch("mlp bwd: reduce");
+}
+
+}  // namespace tfh
+
+using namespace tfh;
+
+extern "C" {
+
+int32_t tfh_feature_mlp_fwd(const tfh_mlp* mlp, const float* features, const float* viewdirs,
+                            const uint32_t* camera_indices, int64_t num_rows, float* rgb,
+                            int32_t precision, tfh_stream_t stream) {
+  TFH_TRY(check_mlp(mlp, "tfh_feature_mlp_fwd"));
+  TFH_REQUIRE(num_rows >= 0, "tfh_feature_mlp_fwd: negative num_rows");
+  TFH_REQUIRE(precision == TFH_PREC_FP32 || precision == TFH_PREC_BF16,
+              "tfh_feature_mlp_fwd: unknown precision %d", precision);
+  if (num_rows == 0) return TFH_OK;
+  TFH_REQUIRE(features && viewdirs && rgb, "tfh_feature_mlp_fwd: null pointer");
+  TFH_REQUIRE(mlp->embed == nullptr || camera_indices,
+              "tfh_feature_mlp_fwd: camera_indices required with an embedding");
+  TFH_TRY(require_device());
+  return mlp_fwd_rows(mlp, features, viewdirs, camera_indices, num_rows, 1, rgb, precision,
+                      as_stream(stream));
+}
+
+size_t tfh_feature_mlp_bwd_workspace_size(const tfh_mlp* mlp, int64_t num_rows) {
+  if (check_mlp(mlp, "tfh_feature_mlp_bwd_workspace_size") != TFH_OK || num_rows < 0) return 0;
+  return mlp_bwd_workspace(mlp, num_rows);
+}
+
+int32_t tfh_feature_mlp_bwd(const tfh_mlp* mlp, const float* features, const float* viewdirs,
+                            const uint32_t* camera_indices, int64_t num_rows,
+                            const float* grad_rgb, float* grad_features, const tfh_mlp_grad* grad,
+                            int32_t precision, void* workspace, size_t workspace_bytes,
+                            tfh_stream_t stream) {
+  TFH_TRY(check_mlp(mlp, "tfh_feature_mlp_bwd"));
+  TFH_REQUIRE(num_rows >= 0, "tfh_feature_mlp_bwd: negative num_rows");
+  TFH_REQUIRE(precision == TFH_PREC_FP32 || precision == TFH_PREC_BF16,
+              "tfh_feature_mlp_bwd: unknown precision %d", precision);
+  if (num_rows == 0) return TFH_OK;
+  TFH_REQUIRE(features && viewdirs && grad_rgb && grad, "tfh_feature_mlp_bwd: null pointer");
+  TFH_REQUIRE(grad->w0 && grad->w1 && grad->b1 && grad->w2 && grad->b2 && grad->w3 && grad->b3,
+              "tfh_feature_mlp_bwd: null gradient pointer");
+  TFH_REQUIRE(mlp->embed == nullptr || (camera_indices && grad->embed),
+              "tfh_feature_mlp_bwd: camera_indices and grad->embed required with an embedding");
+  TFH_TRY(require_device());
+  return mlp_bwd_rows(mlp, features, viewdirs, camera_indices, num_rows, 1, grad_rgb,
+                      grad_features, grad, precision, workspace, workspace_bytes,
+                      as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// Templated VM gather / scatter kernels and their launchers. A point source
+// functor maps a row to its normalised sample point:
+//   PlanarPoints  : ijk (3, M), component stride M (TensorVM.interpolate API)
+//   RaySamples    : pts (R*N, 3) interleaved, row = r*k + j selects sample
+//                   idx[r*k + j] (or j when idx == NULL) of ray r.
+// ROW_MAJOR selects a (M, 3C) feature layout (the MLP input) instead of the
+// reference's (3C, M).
+#pragma once
+
+#include "tfh_vm.h"
+
+namespace tfh {
+
+struct PlanarPoints {
+  const float* p;
+  int64_t m;
+  __device__ __forceinline__ void get(int64_t s, float& x, float& y, float& z) const {
+    x = p[s];
+    y = p[m + s];
+    z = p[2 * m + s];
+  }
+};
+
+struct RaySamples {
+  const float* pts;    // (R*N, 3)
+  const int32_t* idx;  // (R, k) or null
+  int64_t k, n;
+  __device__ __forceinline__ void get(int64_t row, float& x, float& y, float& z) const {
+    const int64_t r = row / k;
+    const int64_t j = row - r * k;
+    const int64_t i = idx ? (int64_t)idx[row] : j;
+    const float* q = pts + (r * n + i) * 3;
+    x = q[0];
+    y = q[1];
+    z = q[2];
+  }
+};
+
+constexpr int kGatherSamples = 16;  // samples per 256-thread block
+
+template <class Src, bool ROW_MAJOR>
+__global__ void __launch_bounds__(256) k_vm_gather(tfh_vm vm, Src src, int64_t M,
+                                                   float* __restrict__ out) {
+  __shared__ float tile[ROW_MAJOR ? 1 : 3 * 64][kGatherSamples + 1];
+  const int grp = threadIdx.x >> 4, l16 = threadIdx.x & 15;
+  const int64_t s0 = (int64_t)blockIdx.x * kGatherSamples;
+  const int64_t s = s0 + grp;
+  const bool active = s < M;
+  const int G = (int)vm.grid_dim, C = (int)vm.channels;
+  float x = 0.f, y = 0.f, z = 0.f;
+  if (active) src.get(s, x, y, z);
+  ModeTaps mt[3];
+  mode_taps(x, y, z, G, mt);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    TapAddr a;
+    tap_addrs(mt[m], m, G, a);
+#pragma unroll
+    for (int q = 0; q < kMaxQ; ++q) {
+      const int c = l16 + 16 * q;
+      if (c < C && active) {
+        float lv, pv;
+        tap_values(vm.plane, vm.line, C, c, a, lv, pv);
+        const float f = lv * pv;  // tensor_vm.py:167
+        if (ROW_MAJOR)
+          out[s * 3 * C + m * C + c] = f;
+        else
+          tile[m * C + c][grp] = f;
+      }
+    }
+  }
+  if (!ROW_MAJOR) {
+    __syncthreads();
+    const int nch = 3 * C;
+    for (int e = threadIdx.x; e < nch * kGatherSamples; e += 256) {
+      const int ch = e / kGatherSamples, j = e % kGatherSamples;
+      if (s0 + j < M) out[(int64_t)ch * M + s0 + j] = tile[ch][j];
+    }
+  }
+}
+
+template <class Src, bool ROW_MAJOR>
+inline int32_t launch_vm_gather(const tfh_vm& vm, const Src& src, int64_t M, float* out,
+                                hipStream_t st) {
+  const int64_t blocks = ceil_div(M, kGatherSamples);
+  TFH_REQUIRE(blocks < (1ll << 31), "vm gather: too many points (%lld)", (long long)M);
+  hipLaunchKernelGGL((k_vm_gather<Src, ROW_MAJOR>), dim3((unsigned)blocks), dim3(256), 0, st, vm,
+                     src, M, out);
+  return check_launch("vm gather");
+}
+
+// VJP: grad (per sample, per channel) -> atomic scatter into the factor
+// gradients. Each wave holds 4 samples; equal texels among them are merged
+// before the atomic (merged_atomic_add).
+template <class Src, bool ROW_MAJOR>
+__global__ void __launch_bounds__(256) k_vm_scatter(tfh_vm vm, Src src, int64_t M,
+                                                    const float* __restrict__ gout,
+                                                    tfh_vm_grad grad) {
+  __shared__ float tile[ROW_MAJOR ? 1 : 3 * 64][kGatherSamples + 1];
+  const int grp = threadIdx.x >> 4, l16 = threadIdx.x & 15;
+  const int64_t s0 = (int64_t)blockIdx.x * kGatherSamples;
+  const int64_t s = s0 + grp;
+  const bool active = s < M;
+  const int G = (int)vm.grid_dim, C = (int)vm.channels;
+  if (!ROW_MAJOR) {
+    const int nch = 3 * C;
+    for (int e = threadIdx.x; e < nch * kGatherSamples; e += 256) {
+      const int ch = e / kGatherSamples, j = e % kGatherSamples;
+      tile[ch][j] = (s0 + j < M) ? gout[(int64_t)ch * M + s0 + j] : 0.0f;
+    }
+    __syncthreads();
+  }
+  float x = 0.f, y = 0.f, z = 0.f;
+  if (active) src.get(s, x, y, z);
+  ModeTaps mt[3];
+  mode_taps(x, y, z, G, mt);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    TapAddr a;
+    tap_addrs(mt[m], m, G, a);
+#pragma unroll
+    for (int q = 0; q < kMaxQ; ++q) {
+      const int c = l16 + 16 * q;
+      if (16 * q < C) {  // wave-uniform: every lane joins the merged atomics
+        const bool on = active && c < C;
+        float lv = 0.f, pv = 0.f, g = 0.f;
+        if (on) {
+          tap_values(vm.plane, vm.line, C, c, a, lv, pv);
+          g = ROW_MAJOR ? gout[s * 3 * C + m * C + c] : tile[m * C + c][grp];
+        }
+        tap_scatter(grad.plane, grad.line, C, c, a, g * pv, g * lv, on);
+      }
+    }
+  }
+}
+
+template <class Src, bool ROW_MAJOR>
+inline int32_t launch_vm_scatter(const tfh_vm& vm, const Src& src, int64_t M, const float* gout,
+                                 const tfh_vm_grad& grad, hipStream_t st) {
+  const int64_t blocks = ceil_div(M, kGatherSamples);
+  TFH_REQUIRE(blocks < (1ll << 31), "vm scatter: too many points (%lld)", (long long)M);
+  hipLaunchKernelGGL((k_vm_scatter<Src, ROW_MAJOR>), dim3((unsigned)blocks), dim3(256), 0, st, vm,
+                     src, M, gout, grad);
+  return check_launch("vm scatter");
+}
+
+}  // namespace tfh

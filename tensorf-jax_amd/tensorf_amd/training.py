@@ -1,0 +1,248 @@
+"""TrainState of the reference (training.py:19-276) over libtensorf_hip.
+
+All learnable parameters live in ONE flat device buffer (density plane, line,
+appearance plane, line, then the MLP leaves), with flat gradient and Adam
+moment buffers of the same layout, so a training step is a single C call
+(tfh_train_step: render -> MSE -> backward -> Adam on every group). The
+reference's functional update is replaced by an in-place one: training_step
+returns the same (mutated) object.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import pathlib
+from typing import Dict, List, Optional, Tuple
+
+import torch
+
+from . import _lib, cameras, networks, render, tensor_vm, utils
+
+
+@dataclasses.dataclass(frozen=True)
+class OptimizerConfig:
+    """train_config.py:6-12."""
+    lr_init_tensor: float = 0.02
+    lr_init_mlp: float = 1e-3
+    lr_decay_iters: Optional[int] = None
+    lr_decay_target_ratio: float = 0.1
+    lr_upsample_reset: bool = True
+
+
+@dataclasses.dataclass(frozen=True)
+class TensorfConfig:
+    """train_config.py:15-71 (dataset/run fields kept for signature parity)."""
+    run_dir: Optional[pathlib.Path] = None
+    dataset_path: Optional[pathlib.Path] = None
+    dataset_type: str = "blender"
+    minibatch_size: int = 4096
+    n_iters: int = 30000
+    optimizer: OptimizerConfig = dataclasses.field(default_factory=OptimizerConfig)
+    initial_aabb_min: Tuple[float, float, float] = (-1.0, -1.0, -1.0)
+    initial_aabb_max: Tuple[float, float, float] = (1.0, 1.0, 1.0)
+    appearance_feat_dim: int = 24
+    density_feat_dim: int = 8
+    feature_n_freqs: int = 6
+    viewdir_n_freqs: int = 6
+    grid_dim_init: int = 128
+    grid_dim_final: int = 300
+    upsamp_iters: Tuple[int, ...] = (2000, 3000, 4000, 5500, 7000)
+    scene_contraction: bool = False
+    scene_scale: float = 1.0
+    camera_embeddings: bool = False
+    render_near: float = 0.05
+    render_far: float = 200.0
+    train_ray_sample_multiplier: float = 1.0
+    # Explicit sample counts and MLP arithmetic (the reference derives the counts
+    # from the grid, training.py:115-118; None keeps that heuristic).
+    density_samples_per_ray: Optional[int] = None
+    appearance_samples_per_ray: Optional[int] = None
+    mlp_precision: str = "fp32"
+
+
+def lr_decay_coeff(step: int, config: TensorfConfig) -> float:
+    """training.py:161-181."""
+    opt = config.optimizer
+    if opt.lr_upsample_reset:
+        s = min(step - u for u in (0,) + tuple(config.upsamp_iters) if step - u >= 0)
+    else:
+        s = step
+    decay_iters = opt.lr_decay_iters if opt.lr_decay_iters is not None else config.n_iters
+    v = opt.lr_decay_target_ratio ** (s / decay_iters)
+    return max(v, opt.lr_decay_target_ratio)
+
+
+def sample_counts(config: TensorfConfig, grid_dim: int) -> Tuple[int, int]:
+    """training.py:115-118, unless the config pins the counts."""
+    n = config.density_samples_per_ray or int(
+        math.sqrt(3 * grid_dim ** 2) * config.train_ray_sample_multiplier)
+    k = config.appearance_samples_per_ray or int(0.15 * n)
+    return n, max(1, min(k, n))
+
+
+class FlatParams:
+    """One flat buffer holding every learnable parameter, plus views."""
+
+    def __init__(self, shapes: List[Tuple[str, Tuple[int, ...]]], tensor_groups: int, device):
+        self.names = [n for n, _ in shapes]
+        self.shapes = [s for _, s in shapes]
+        sizes = [math.prod(s) for s in self.shapes]
+        # 64-float alignment keeps every view 256-byte aligned
+        self.offsets = []
+        off = 0
+        for sz in sizes:
+            self.offsets.append(off)
+            off += (sz + 63) // 64 * 64
+        self.total = off
+        self.tensor_count = self.offsets[tensor_groups]
+        self.params = torch.zeros(self.total, dtype=torch.float32, device=device)
+        self.grads = torch.zeros_like(self.params)
+        self.exp_avg = torch.zeros_like(self.params)
+        self.exp_avg_sq = torch.zeros_like(self.params)
+
+    def view(self, buf: torch.Tensor, i: int) -> torch.Tensor:
+        o = self.offsets[i]
+        return buf[o:o + math.prod(self.shapes[i])].view(self.shapes[i])
+
+    def c_struct(self) -> _lib.TrainBuffers:
+        return _lib.TrainBuffers(_lib.ptr(self.params), _lib.ptr(self.grads),
+                                 _lib.ptr(self.exp_avg), _lib.ptr(self.exp_avg_sq), self.total,
+                                 self.tensor_count)
+
+
+class TrainState:
+    """training.py:19-276."""
+
+    def __init__(self, config: TensorfConfig, appearance_mlp: networks.FeatureMlp,
+                 flat: FlatParams, aabb: torch.Tensor, prng_key: int, step: int = 0,
+                 adam_count: int = 0):
+        self.config = config
+        self.appearance_mlp = appearance_mlp
+        self.flat = flat
+        self.aabb = aabb
+        self.prng_key = prng_key
+        self.step = step
+        self.adam_count = adam_count
+        self._ws = None
+        self._mse = torch.zeros(1, dtype=torch.float32, device=aabb.device)
+
+    # ---- construction -----------------------------------------------------------
+    @staticmethod
+    def initialize(config: TensorfConfig, grid_dim: int, prng_key: int, num_cameras: int,
+                   device="cuda") -> "TrainState":
+        """training.py:35-99."""
+        gen = torch.Generator().manual_seed(int(prng_key))
+        mlp = networks.FeatureMlp(feature_n_freqs=config.feature_n_freqs,
+                                  viewdir_n_freqs=config.viewdir_n_freqs,
+                                  num_cameras=num_cameras if config.camera_embeddings else None,
+                                  precision=config.mlp_precision)
+        ca, cd = config.appearance_feat_dim, config.density_feat_dim
+        mparams = mlp.init(gen, torch.zeros((1, 3 * ca), device=device))
+        app = tensor_vm.TensorVM.initialize(grid_dim, ca, prng_key=gen, device=device)
+        den = tensor_vm.TensorVM.initialize(grid_dim, cd, prng_key=gen, device=device)
+        aabb = torch.tensor([config.initial_aabb_min, config.initial_aabb_max],
+                            dtype=torch.float32, device=device)
+        state = TrainState(config, mlp, None, aabb, int(prng_key))
+        state._load(den, app, mparams)
+        return state
+
+    def _load(self, den: tensor_vm.TensorVM, app: tensor_vm.TensorVM, mparams: Dict,
+              moments=None):
+        leaves = networks.FeatureMlp.flat_leaves(mparams)
+        names = ["density_plane", "density_line", "appearance_plane", "appearance_line",
+                 "w0", "w1", "b1", "w2", "b2", "w3", "b3", "embed"][:4 + len(leaves)]
+        tensors = [den.stacked_single_vm.plane, den.stacked_single_vm.line,
+                   app.stacked_single_vm.plane, app.stacked_single_vm.line, *leaves]
+        flat = FlatParams([(n, tuple(t.shape)) for n, t in zip(names, tensors)], 4,
+                          self.aabb.device)
+        for i, t in enumerate(tensors):
+            flat.view(flat.params, i).copy_(t)
+        if moments is not None:
+            for i, (m, v) in enumerate(moments):
+                flat.view(flat.exp_avg, i).copy_(m)
+                flat.view(flat.exp_avg_sq, i).copy_(v)
+        self.flat = flat
+        self._ws = None
+
+    # ---- views in the reference's structure -----------------------------------------
+    def _vm(self, buf, i) -> tensor_vm.TensorVM:
+        return tensor_vm.TensorVM(tensor_vm.TensorVMSingle(self.flat.view(buf, i),
+                                                           self.flat.view(buf, i + 1)))
+
+    def _mlp_params(self, buf) -> Dict:
+        f = self.flat
+        p = {"Dense_0": {"kernel": f.view(buf, 4)},
+             "Dense_1": {"kernel": f.view(buf, 5), "bias": f.view(buf, 6)},
+             "Dense_2": {"kernel": f.view(buf, 7), "bias": f.view(buf, 8)},
+             "Dense_3": {"kernel": f.view(buf, 9), "bias": f.view(buf, 10)}}
+        if len(f.names) > 11:
+            p["Embed_0"] = {"embedding": f.view(buf, 11)}
+        return {"params": p}
+
+    @property
+    def learnable_params(self) -> render.LearnableParams:
+        return render.LearnableParams(self._mlp_params(self.flat.params),
+                                      self._vm(self.flat.params, 2), self._vm(self.flat.params, 0),
+                                      self.config.scene_contraction)
+
+    def render_config(self, mode=render.RenderMode.RGB) -> render.RenderConfig:
+        n, k = sample_counts(self.config, self.flat.shapes[0][1])
+        return render.RenderConfig(self.config.render_near, self.config.render_far, mode, n, k)
+
+    # ---- training ------------------------------------------------------------------
+    def training_step(self, minibatch: cameras.RenderedRays) -> Tuple["TrainState", Dict]:
+        """training.py:101-205 as one library call. Logs hold device scalars."""
+        rays = minibatch.rays_wrt_world.reshape_flat()
+        colors = minibatch.colors.reshape(-1, 3).to(torch.float32).contiguous()
+        lp = self.learnable_params
+        cfg = self.render_config()
+        env = render._RenderEnv(self.appearance_mlp, lp, self.aabb, rays,
+                                render._seed(self.prng_key, self.step), cfg, self._ws)
+        self._ws = env.ws  # reused by later steps of the same shape
+        grads_lp = (self._vm(self.flat.grads, 0), self._vm(self.flat.grads, 2),
+                    self._mlp_params(self.flat.grads))
+        dg = _lib.VmGrad(_lib.ptr(grads_lp[0].stacked_single_vm.plane),
+                         _lib.ptr(grads_lp[0].stacked_single_vm.line))
+        ag = _lib.VmGrad(_lib.ptr(grads_lp[1].stacked_single_vm.plane),
+                         _lib.ptr(grads_lp[1].stacked_single_vm.line))
+        leaves = networks.FeatureMlp.flat_leaves(grads_lp[2])
+        mg = _lib.MlpGrad(*[_lib.ptr(t) for t in leaves], *([None] * (8 - len(leaves))))
+        bufs = self.flat.c_struct()
+        decay = lr_decay_coeff(self.step, self.config)
+        self.adam_count += 1
+        _lib.call("tfh_train_step", ctypes.byref(env.dvm), ctypes.byref(env.avm),
+                  ctypes.byref(env.cmlp), ctypes.byref(env.cfg), _lib.ptr(env.aabb),
+                  _lib.ptr(env.o), _lib.ptr(env.d), _lib.ptr(env.cams), _lib.ptr(colors), env.r,
+                  _lib.ptr(env.jitter), _lib.ptr(env.base_ts), _lib.ptr(env.base_steps),
+                  _lib.ptr(env.gumbel), ctypes.byref(dg), ctypes.byref(ag), ctypes.byref(mg),
+                  ctypes.byref(bufs), self.config.optimizer.lr_init_tensor,
+                  self.config.optimizer.lr_init_mlp, decay, self.adam_count, _lib.ptr(self._mse),
+                  _lib.ptr(env.ws), env.ws.numel(), _lib.stream())
+        self.step += 1
+        logs = {"mse": self._mse, "lr_tensor": decay * self.config.optimizer.lr_init_tensor,
+                "lr_mlp": decay * self.config.optimizer.lr_init_mlp}
+        return self, logs
+
+    def resize_grid(self, new_grid_dim: int) -> "TrainState":
+        """training.py:245-276: resample the factors AND the Adam moments."""
+        f = self.flat
+        out = {}
+        for name, buf in (("p", f.params), ("m", f.exp_avg), ("v", f.exp_avg_sq)):
+            out[name] = (self._vm(buf, 0).resize(new_grid_dim),
+                         self._vm(buf, 2).resize(new_grid_dim))
+        mparams = {k: {n: t.clone() for n, t in v.items()}
+                   for k, v in self._mlp_params(f.params)["params"].items()}
+        mlp_m = [t.clone() for t in networks.FeatureMlp.flat_leaves(self._mlp_params(f.exp_avg))]
+        mlp_v = [t.clone() for t in networks.FeatureMlp.flat_leaves(self._mlp_params(f.exp_avg_sq))]
+        moments = []
+        for j in (0, 1):
+            for vm_m, vm_v in ((out["m"][j], out["v"][j]),):
+                moments.append((vm_m.stacked_single_vm.plane, vm_v.stacked_single_vm.plane))
+                moments.append((vm_m.stacked_single_vm.line, vm_v.stacked_single_vm.line))
+        moments += list(zip(mlp_m, mlp_v))
+        self._load(out["p"][0], out["p"][1], {"params": mparams}, moments)
+        return self
+
+    def psnr(self) -> float:
+        return utils.psnr_from_mse(self._mse.item())
