@@ -57,6 +57,10 @@ def stage_model(stage, R, N, k, cd, ca, precision, n_params):
     return None
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def make_batches(R, count, seed, device):
     import torch
     g = torch.Generator(device="cpu").manual_seed(seed)
@@ -114,8 +118,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the step would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the training step has no multi-GPU path "
+              "(data parallelism over rays is a design note, not code). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -129,6 +134,7 @@ def main():
         initial_aabb_max=(1.5,) * 3, appearance_feat_dim=ca, density_feat_dim=cd,
         feature_n_freqs=2, viewdir_n_freqs=2, grid_dim_init=G, grid_dim_final=G, upsamp_iters=(),
         density_samples_per_ray=N, appearance_samples_per_ray=k, mlp_precision=args.precision)
+    progress(f"init state G={G} R={R} N={N} k={k} {args.precision}")
     state = training.TrainState.initialize(cfg, G, prng_key=0, num_cameras=1)
     batches = make_batches(R, 4, seed=1, device="cuda")
     cams = torch.zeros(R, dtype=torch.int32, device="cuda")
@@ -139,13 +145,15 @@ def main():
 
     for i in range(args.warmup):
         step(i)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        progress(f"warmup step {i} done")
     t0 = time.perf_counter()
     for i in range(args.steps):
         step(i)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = R * args.steps / dt
+    progress(f"timed {args.steps} steps: {1e3 * dt / args.steps:.3f} ms/step")
     mse = float(state._mse.item())
 
     # render rays/s (RGB forward, no gradient) on the same batches
@@ -168,6 +176,7 @@ def main():
         render_once(i)
     torch.cuda.synchronize()
     render_value = R * n_render / (time.perf_counter() - t1)
+    progress(f"render: {render_value:.1f} rays/s")
 
     # live per-stage timing (HIP events on the launch stream) of a few steps
     stages = {}
@@ -180,6 +189,7 @@ def main():
         torch.cuda.synchronize()
         rec = _lib.profile_read()
         _lib.profile_enable(False)
+        progress("stages: " + ", ".join(f"{k}={v[0] / max(v[1], 1):.3f}ms" for k, v in rec.items()))
         n_params = state.flat.total
         best = None
         for name, (ms, cnt) in rec.items():
@@ -210,6 +220,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        progress(f"cpu baseline on {args.cpu_sample_rays} rays")
         cpu = cpu_baseline(args.cpu_sample_rays, N, k, G, cd, ca)
 
     line = {

@@ -160,13 +160,17 @@ int32_t tfh_topk_select(const float* p_terminates, int64_t num_rays, int64_t sam
                         int64_t k, const float* gumbel, int32_t* indices, tfh_stream_t stream);
 
 /* ---- FeatureMlp --------------------------------------------------------- */
+/* Workspace of tfh_feature_mlp_fwd (0 for FP32; BF16 packs the weights). */
+size_t tfh_feature_mlp_fwd_workspace_size(const tfh_mlp* mlp, int32_t precision);
 /* FeatureMlp.__call__ (networks.py:46-121): features (M, feat_dim),
  * viewdirs (M, 3), camera_indices (M) (may be NULL without embedding),
  * rgb (M, 3). */
 int32_t tfh_feature_mlp_fwd(const tfh_mlp* mlp, const float* features, const float* viewdirs,
                             const uint32_t* camera_indices, int64_t num_rows, float* rgb,
-                            int32_t precision, tfh_stream_t stream);
-size_t tfh_feature_mlp_bwd_workspace_size(const tfh_mlp* mlp, int64_t num_rows);
+                            int32_t precision, void* workspace, size_t workspace_bytes,
+                            tfh_stream_t stream);
+size_t tfh_feature_mlp_bwd_workspace_size(const tfh_mlp* mlp, int64_t num_rows,
+                                          int32_t precision);
 /* VJP of tfh_feature_mlp_fwd: grad_rgb (M,3) -> grad_features (M, feat_dim)
  * (may be NULL) and parameter gradients accumulated into grad. */
 int32_t tfh_feature_mlp_bwd(const tfh_mlp* mlp, const float* features, const float* viewdirs,

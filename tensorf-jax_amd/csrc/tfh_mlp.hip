@@ -8,61 +8,83 @@
 //   h2[U/2:] = e[:U/2] * h2[U/2:] + e[U/2:],  e = embed[cam]  networks.py:103-111
 //   rgb = sigmoid(h2 @ W3 + b3)       (U -> 3)               networks.py:114-120
 //
-// A workgroup owns a tile of TM rows. Activations live in LDS (row-major,
-// padded strides that keep MFMA operand reads bank-conflict free); every
-// layer is a set of 16x16 output tiles computed with
-// v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f32_16x16x32_bf16 (BF16 operands,
-// FP32 accumulation). Weights are read from global memory (L2-resident).
-//
-// The backward recomputes the forward of its tile and keeps weight gradients
-// in a per-workgroup partial buffer (plain read-modify-write, each partial
-// tile owned by one wave), reduced across workgroups by k_mlp_reduce.
+// A workgroup owns a tile of TM rows. Activations live in LDS (FP32,
+// row-major, padded strides); every layer is a set of 16x16 output tiles:
+//   FP32: v_mfma_f32_16x16x4_f32, weights read from global (L2-resident);
+//   BF16: v_mfma_f32_16x16x32_bf16 with FP32 accumulation; activations are
+//         rounded to BF16 when loaded as operands, weights are packed once per
+//         call into BF16 [out][in] / [in][out] tables so each lane's 8
+//         operands are one 16-byte load.
+// The backward recomputes the forward of its tile and accumulates weight
+// gradients into a per-workgroup partial buffer (plain read-modify-write,
+// each partial tile owned by one wave), reduced across workgroups by
+// k_mlp_reduce.
+#include <algorithm>
+
 #include "tfh_common.h"
 
 namespace tfh {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kU = 128;   // hidden width
 constexpr int kSP = 32;   // padded squash width
 
 struct MlpDims {
-  int F, FP;   // feature dim, padded to 16
+  int F, FP;   // feature dim, padded
   int S;       // squash dim (<= 32)
-  int E, EP;   // encoded dim, padded to 16
+  int E, EP;   // encoded dim, padded
   int nf, nv;
   int ncam;
+  int Z3W;     // padded width of the layer-3 output / its gradient
   // LDS strides (floats)
   int ldF, ldE, ldU, ldS, ld3;
 };
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-inline MlpDims make_dims(const tfh_mlp* p) {
+// FP32 pads K to 16 with strides = 2 mod 32 floats (conflict-free scalar
+// reads); BF16 pads K to 32 with strides = 4 mod 32 (16-byte aligned rows,
+// conflict-free 128-bit reads).
+inline MlpDims make_dims(const tfh_mlp* p, bool bf) {
   MlpDims d;
+  const int kp = bf ? 32 : 16;
+  const int pad = bf ? 4 : 2;
   d.F = (int)p->feat_dim;
-  d.FP = round_up(d.F, 16);
+  d.FP = round_up(d.F, kp);
   d.S = (int)p->squash_dim;
   d.nf = (int)p->feature_n_freqs;
   d.nv = (int)p->viewdir_n_freqs;
   d.E = d.S + 3 + 2 * d.nf * d.S + 2 * d.nv * 3;
-  d.EP = round_up(d.E, 16);
+  d.EP = round_up(d.E, kp);
   d.ncam = p->embed ? (int)p->num_cameras : 0;
-  d.ldF = d.FP + 2;
-  d.ldE = d.EP + 2;
-  d.ldU = kU + 2;
-  d.ldS = kSP + 2;
-  d.ld3 = 16 + 2;
+  d.Z3W = bf ? 32 : 16;
+  d.ldF = d.FP + pad;
+  d.ldE = d.EP + pad;
+  d.ldU = kU + pad;
+  d.ldS = kSP + pad;
+  d.ld3 = d.Z3W + pad;
   return d;
 }
 
-struct MlpW {  // device pointers of the parameters
+struct MlpW {  // FP32 parameters (device pointers)
   const float *w0, *w1, *b1, *w2, *b2, *w3, *b3, *embed;
+};
+
+// Operand tables built per call in the workspace.
+struct MlpPack {
+  // FP32 mode: transposed weights for the input-gradient GEMMs
+  const float *w0t, *w1t, *w2t, *w3t;  // (S,F), (U,E), (U,U), (3,U)
+  // BF16 mode: forward tables [out][in] and backward tables [in][out]
+  const __bf16 *f0, *f1, *f2, *f3;     // [32][FP], [U][EP], [U][U], [16][U]
+  const __bf16 *g0, *g1, *g2, *g3;     // [FP][32], [EP][U], [U][U], [U][32]
 };
 
 // ---- MFMA tile helpers --------------------------------------------------------
 // D layout of a 16x16 tile: lane l, element r -> row 4*(l/16)+r, col l%16.
-// A operand: lane l holds A[l%16][k0 + l/16]; B operand: B[k0 + l/16][l%16].
+// FP32 (K=4):  A[l%16][k0 + l/16],        B[k0 + l/16][l%16]
+// BF16 (K=32): A[l%16][k0 + 8(l/16) + e], B[k0 + 8(l/16) + e][l%16], e < 8
 
 template <int NR>
 __device__ __forceinline__ void zero_acc(f32x4 (&acc)[NR]) {
@@ -70,42 +92,14 @@ __device__ __forceinline__ void zero_acc(f32x4 (&acc)[NR]) {
   for (int i = 0; i < NR; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 }
 
-// acc[rt] += A(rows 16*rt.., K) * B(K, 16 cols), rt < NR; A from LDS
-// (row-major, lda), B(k, j) = bsrc(k, j).
-template <int NR, class BSrc>
-__device__ __forceinline__ void mma_strip_f32(const float* __restrict__ A, int lda, int K,
-                                              BSrc bsrc, int col0, f32x4 (&acc)[NR]) {
-  const int lane = lane_id();
-  const int r = lane & 15, kq = lane >> 4;
-#pragma unroll 4
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const float b = bsrc(k0 + kq, col0 + r);
-#pragma unroll
-    for (int rt = 0; rt < NR; ++rt) {
-      const float a = A[(16 * rt + r) * lda + k0 + kq];
-      acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[rt], 0, 0, 0);
-    }
-  }
+__device__ __forceinline__ bf16x8 to_bf16x8(const float4& lo, const float4& hi) {
+  bf16x8 v;
+  v[0] = (__bf16)lo.x; v[1] = (__bf16)lo.y; v[2] = (__bf16)lo.z; v[3] = (__bf16)lo.w;
+  v[4] = (__bf16)hi.x; v[5] = (__bf16)hi.y; v[6] = (__bf16)hi.z; v[7] = (__bf16)hi.w;
+  return v;
 }
 
-// acc += A^T-style product for weight gradients:
-// acc(i, j) += sum_k X[k][row0 + i] * Y[k][col0 + j], k < K (rows of the tile)
-__device__ __forceinline__ f32x4 mma_wgrad_f32(const float* __restrict__ X, int ldx, int row0,
-                                               const float* __restrict__ Y, int ldy, int col0,
-                                               int K) {
-  const int lane = lane_id();
-  const int r = lane & 15, kq = lane >> 4;
-  f32x4 acc{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const float a = X[(k0 + kq) * ldx + row0 + r];
-    const float b = Y[(k0 + kq) * ldy + col0 + r];
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-  }
-  return acc;
-}
-
-// Row-major global matrix as a B operand with zero padding outside (rows, cols).
+// Row-major FP32 global matrix as a B operand with zero padding outside (rows, cols).
 struct GlobalB {
   const float* w;
   int rows, cols;
@@ -114,9 +108,122 @@ struct GlobalB {
   }
 };
 
+// BF16 table T[j][k] (stride ld) as a B operand: B(k, j) = T[j][k].
+struct PackedB {
+  const __bf16* t;
+  int ld;
+  __device__ __forceinline__ bf16x8 load8(int k, int j) const {
+    return *reinterpret_cast<const bf16x8*>(t + (size_t)j * ld + k);
+  }
+};
+
+// acc[rt] += A(rows 16*rt.., K) * B(K, cols col0..col0+15), rt < NR; A from
+// LDS (row-major, lda).
+template <int NR>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, GlobalB b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  const int lane = lane_id();
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll 4
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const float bv = b(k0 + kq, col0 + r);
+#pragma unroll
+    for (int rt = 0; rt < NR; ++rt) {
+      const float a = A[(16 * rt + r) * lda + k0 + kq];
+      acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc[rt], 0, 0, 0);
+    }
+  }
+}
+
+template <int NR>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  const int lane = lane_id();
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll 2
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const bf16x8 bv = b.load8(k0 + 8 * kq, col0 + r);
+#pragma unroll
+    for (int rt = 0; rt < NR; ++rt) {
+      const float* ap = A + (16 * rt + r) * lda + k0 + 8 * kq;
+      const bf16x8 av = to_bf16x8(*reinterpret_cast<const float4*>(ap),
+                                  *reinterpret_cast<const float4*>(ap + 4));
+      acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[rt], 0, 0, 0);
+    }
+  }
+}
+
+// Weight-gradient tile: acc(i, j) = sum_k X[k][row0 + i] * Y[k][col0 + j],
+// k < K (the rows of the tile).
+template <bool BF>
+__device__ __forceinline__ f32x4 mma_wgrad(const float* __restrict__ X, int ldx, int row0,
+                                           const float* __restrict__ Y, int ldy, int col0, int K) {
+  const int lane = lane_id();
+  const int r = lane & 15, kq = lane >> 4;
+  f32x4 acc{0.f, 0.f, 0.f, 0.f};
+  if constexpr (BF) {
+    for (int k0 = 0; k0 < K; k0 += 32) {
+      bf16x8 av, bv;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        av[e] = (__bf16)X[(k0 + 8 * kq + e) * ldx + row0 + r];
+        bv[e] = (__bf16)Y[(k0 + 8 * kq + e) * ldy + col0 + r];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+  } else {
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const float a = X[(k0 + kq) * ldx + row0 + r];
+      const float b = Y[(k0 + kq) * ldy + col0 + r];
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+  }
+  return acc;
+}
+
+// B operands of each GEMM, by precision.
+template <bool BF>
+struct Operands;
+
+template <>
+struct Operands<false> {
+  const MlpDims& d;
+  const MlpW& w;
+  const MlpPack& p;
+  __device__ GlobalB l0() const { return GlobalB{w.w0, d.F, d.S}; }
+  __device__ GlobalB l1() const { return GlobalB{w.w1, d.E, kU}; }
+  __device__ GlobalB l2() const { return GlobalB{w.w2, kU, kU}; }
+  __device__ GlobalB l3() const { return GlobalB{w.w3, kU, 3}; }
+  __device__ GlobalB g3() const { return GlobalB{p.w3t, 3, kU}; }
+  __device__ GlobalB g2() const { return GlobalB{p.w2t, kU, kU}; }
+  __device__ GlobalB g1() const { return GlobalB{p.w1t, kU, d.E}; }
+  __device__ GlobalB g0() const { return GlobalB{p.w0t, d.S, d.F}; }
+  __device__ int k0() const { return round_up(d.F, 4); }
+  __device__ int k3() const { return 4; }  // K of dZ3 @ W3^T (3 padded to 4)
+};
+
+template <>
+struct Operands<true> {
+  const MlpDims& d;
+  const MlpW& w;
+  const MlpPack& p;
+  __device__ PackedB l0() const { return PackedB{p.f0, d.FP}; }
+  __device__ PackedB l1() const { return PackedB{p.f1, d.EP}; }
+  __device__ PackedB l2() const { return PackedB{p.f2, kU}; }
+  __device__ PackedB l3() const { return PackedB{p.f3, kU}; }
+  __device__ PackedB g3() const { return PackedB{p.g3, 32}; }
+  __device__ PackedB g2() const { return PackedB{p.g2, kU}; }
+  __device__ PackedB g1() const { return PackedB{p.g1, kU}; }
+  __device__ PackedB g0() const { return PackedB{p.g0, 32}; }
+  __device__ int k0() const { return d.FP; }
+  __device__ int k3() const { return 32; }
+};
+
 // ---- shared forward pieces ------------------------------------------------------
 // Loads TM rows of features (zero padded to FP columns and past M), view
-// directions and camera indices of the tile.
+// directions and camera indices of the tile. Row `row` belongs to ray
+// row / rpr (view direction and camera index are per ray).
 template <int TM>
 __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
                           const float* __restrict__ dirs, const uint32_t* __restrict__ cams,
@@ -173,10 +280,10 @@ __device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
 // Runs layers 0..2 of the tile. On exit: X holds the encoded input, H0 (if
 // non-null) a copy of h0, H1 = relu(z1), H2p = relu(z2) (pre-FiLM) and H2 the
 // layer-3 input (== H2p without embedding). Fs is read by layer 0 only.
-template <int TM, int NW>
-__device__ void forward_hidden(const MlpDims& d, const MlpW& w, const float* Fs, float* X,
-                               float* H0, float* H1, float* H2p, float* H2, const float* Ds,
-                               const int* Cs) {
+template <int TM, int NW, bool BF>
+__device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
+                               const float* Fs, float* X, float* H0, float* H1, float* H2p,
+                               float* H2, const float* Ds, const int* Cs) {
   constexpr int RT = TM / 16;
   const int wave = threadIdx.x >> 6;
   const int lane = lane_id();
@@ -184,7 +291,7 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const float* Fs,
   for (int ct = wave; ct < 2; ct += NW) {
     f32x4 acc[RT];
     zero_acc(acc);
-    mma_strip_f32<RT>(Fs, d.ldF, round_up(d.F, 4), GlobalB{w.w0, d.F, d.S}, 16 * ct, acc);
+    mma_strip<RT>(Fs, d.ldF, op.k0(), op.l0(), 16 * ct, acc);
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -203,7 +310,7 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const float* Fs,
   for (int ct = wave; ct < kU / 16; ct += NW) {
     f32x4 acc[RT];
     zero_acc(acc);
-    mma_strip_f32<RT>(X, d.ldE, d.EP, GlobalB{w.w1, d.E, kU}, 16 * ct, acc);
+    mma_strip<RT>(X, d.ldE, d.EP, op.l1(), 16 * ct, acc);
     const int col = 16 * ct + (lane & 15);
     const float bias = w.b1[col];
 #pragma unroll
@@ -219,7 +326,7 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const float* Fs,
   for (int ct = wave; ct < kU / 16; ct += NW) {
     f32x4 acc[RT];
     zero_acc(acc);
-    mma_strip_f32<RT>(H1, d.ldU, kU, GlobalB{w.w2, kU, kU}, 16 * ct, acc);
+    mma_strip<RT>(H1, d.ldU, kU, op.l2(), 16 * ct, acc);
     const int col = 16 * ct + (lane & 15);
     const float bias = w.b2[col];
 #pragma unroll
@@ -240,18 +347,17 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const float* Fs,
   __syncthreads();
 }
 
-// layer 3 + sigmoid: rgb[row][c] for c < 3, rows of the tile. Writes
-// sigmoid values into R3 (TM x ld3, columns >= 3 untouched).
-template <int TM, int NW>
-__device__ void forward_out(const MlpDims& d, const MlpW& w, const float* H2, float* R3) {
+// layer 3 + sigmoid: R3[row][c] for c < 3 (other columns untouched).
+template <int TM, int NW, bool BF>
+__device__ void forward_out(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
+                            const float* H2, float* R3) {
   constexpr int RT = TM / 16;
   const int wave = threadIdx.x >> 6;
   const int lane = lane_id();
-  // one 16-wide column tile; distribute row tiles over waves
   for (int rt = wave; rt < RT; rt += NW) {
     f32x4 acc[1];
     zero_acc(acc);
-    mma_strip_f32<1>(H2 + 16 * rt * d.ldU, d.ldU, kU, GlobalB{w.w3, kU, 3}, 0, acc);
+    mma_strip<1>(H2 + 16 * rt * d.ldU, d.ldU, kU, op.l3(), 0, acc);
     const int col = lane & 15;
     if (col < 3) {
       const float bias = w.b3[col];
@@ -267,42 +373,38 @@ __device__ void forward_out(const MlpDims& d, const MlpW& w, const float* H2, fl
 }
 
 // ---- forward kernel -----------------------------------------------------------
-template <int TM, int NW>
-__global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d, MlpW w,
+template <int TM>
+inline size_t fwd_lds_floats(const MlpDims& d) {
+  return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + d.ld3 + 4);
+}
+
+template <int TM, int NW, bool BF>
+__global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d, MlpW w, MlpPack pk,
                                                      const float* __restrict__ feats,
                                                      const float* __restrict__ dirs,
                                                      const uint32_t* __restrict__ cams,
                                                      int64_t M, int64_t rpr,
                                                      float* __restrict__ rgb) {
   extern __shared__ float smem[];
-  // regions: A = max(Fs, H1), B = max(X, H2), small: Ds, Cs, R3
+  // regions: A = Fs, later H1; B = X, later H2; small: Ds, R3, Cs
   float* regA = smem;
-  float* regB = regA + TM * (d.ldF > d.ldU ? d.ldF : d.ldU);
-  float* Ds = regB + TM * (d.ldE > d.ldU ? d.ldE : d.ldU);
-  float* R3 = Ds + TM * 3;
+  float* regB = regA + TM * std::max(d.ldF, d.ldU);
+  float* Ds = regB + TM * std::max(d.ldE, d.ldU);
+  float* R3 = Ds + TM * 4;
   int* Cs = reinterpret_cast<int*>(R3 + TM * d.ld3);
+  const Operands<BF> op{d, w, pk};
   const int64_t row0 = (int64_t)blockIdx.x * TM;
   load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, regA, Ds, Cs);
   __syncthreads();
-  // Fs = regA, X = regB, H1 = regA (after layer 0), H2 = regB (after layer 1)
-  forward_hidden<TM, NW>(d, w, regA, regB, nullptr, regA, regB, regB, Ds, Cs);
-  forward_out<TM, NW>(d, w, regB, R3);
+  // layer 1 reads X (regB) and writes H1 over Fs (regA, dead after layer 0);
+  // layer 2 reads H1 and writes H2 over X (dead after layer 1)
+  forward_hidden<TM, NW, BF>(d, w, op, regA, regB, nullptr, regA, regB, regB, Ds, Cs);
+  forward_out<TM, NW, BF>(d, w, op, regB, R3);
   for (int e = threadIdx.x; e < TM * 3; e += blockDim.x) {
     const int i = e / 3, c = e % 3;
     const int64_t row = row0 + i;
     if (row < M) rgb[row * 3 + c] = R3[i * d.ld3 + c];
   }
-}
-
-// NOTE: forward_hidden reads Fs in layer 0 and writes H1 in layer 1 into the
-// same region; layer 0 completes (barrier) before layer 1 starts. X and H2
-// share a region: layer 2 reads H1 and writes H2 after layer 1 consumed X.
-
-template <int TM>
-inline size_t fwd_lds_bytes(const MlpDims& d) {
-  size_t a = (size_t)TM * (d.ldF > d.ldU ? d.ldF : d.ldU);
-  size_t b = (size_t)TM * (d.ldE > d.ldU ? d.ldE : d.ldU);
-  return (a + b + TM * 3 + TM * d.ld3 + TM) * sizeof(float);
 }
 
 // ---- backward ---------------------------------------------------------------------
@@ -325,30 +427,65 @@ inline GradPack make_pack(const MlpDims& d) {
   return p;
 }
 
-// Transposed copies of the weights (B operands of the input-gradient GEMMs).
-struct MlpWT {
-  float *w0t, *w1t, *w2t, *w3t;  // (S,F), (U,E), (U,U), (3,U)
+// Operand tables written by k_pack_weights. One thread per table entry;
+// entries outside the real matrix are zero.
+struct PackOut {
+  float *w0t, *w1t, *w2t, *w3t;
+  __bf16 *f0, *f1, *f2, *f3, *g0, *g1, *g2, *g3;
 };
 
-__global__ void k_transpose_weights(MlpDims d, MlpW w, MlpWT t) {
-  const int64_t n0 = (int64_t)d.F * d.S, n1 = (int64_t)d.E * kU, n2 = (int64_t)kU * kU,
-                n3 = (int64_t)kU * 3;
-  const int64_t total = n0 + n1 + n2 + n3;
+struct PackSizes {
+  int64_t t0, t1, t2, t3;      // FP32 transposes
+  int64_t f0, f1, f2, f3;      // BF16 forward tables
+  int64_t g0, g1, g2, g3;      // BF16 backward tables
+};
+
+inline PackSizes pack_sizes(const MlpDims& d) {
+  PackSizes s;
+  s.t0 = (int64_t)d.S * d.F; s.t1 = (int64_t)kU * d.E; s.t2 = kU * kU; s.t3 = 3 * kU;
+  s.f0 = (int64_t)32 * d.FP; s.f1 = (int64_t)kU * d.EP; s.f2 = kU * kU; s.f3 = 16 * kU;
+  s.g0 = (int64_t)d.FP * 32; s.g1 = (int64_t)d.EP * kU; s.g2 = kU * kU; s.g3 = kU * 32;
+  return s;
+}
+
+__device__ __forceinline__ float wat(const float* w, int rows, int cols, int r, int c) {
+  return (r < rows && c < cols) ? w[r * cols + c] : 0.0f;
+}
+
+template <bool BF>
+__global__ void k_pack_weights(MlpDims d, MlpW w, PackOut o, PackSizes s) {
+  const int64_t total = BF ? s.f0 + s.f1 + s.f2 + s.f3 + s.g0 + s.g1 + s.g2 + s.g3
+                           : s.t0 + s.t1 + s.t2 + s.t3;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
-    if (e < n0) {  // w0 (F, S) -> w0t (S, F)
-      const int64_t i = e / d.S, j = e % d.S;
-      t.w0t[j * d.F + i] = w.w0[e];
-    } else if (e < n0 + n1) {
-      const int64_t f = e - n0, i = f / kU, j = f % kU;
-      t.w1t[j * d.E + i] = w.w1[f];
-    } else if (e < n0 + n1 + n2) {
-      const int64_t f = e - n0 - n1, i = f / kU, j = f % kU;
-      t.w2t[j * kU + i] = w.w2[f];
-    } else {
-      const int64_t f = e - n0 - n1 - n2, i = f / 3, j = f % 3;
-      t.w3t[j * kU + i] = w.w3[f];
+    int64_t f = e;
+    if (!BF) {
+      if (f < s.t0) { const int j = f / d.F, i = f % d.F; o.w0t[f] = w.w0[i * d.S + j]; continue; }
+      f -= s.t0;
+      if (f < s.t1) { const int j = f / d.E, i = f % d.E; o.w1t[f] = w.w1[i * kU + j]; continue; }
+      f -= s.t1;
+      if (f < s.t2) { const int j = f / kU, i = f % kU; o.w2t[f] = w.w2[i * kU + j]; continue; }
+      f -= s.t2;
+      { const int j = f / kU, i = f % kU; o.w3t[f] = w.w3[i * 3 + j]; }
+      continue;
     }
+    // forward tables T[j][k] = W[k][j]
+    if (f < s.f0) { const int j = f / d.FP, k = f % d.FP; o.f0[f] = (__bf16)wat(w.w0, d.F, d.S, k, j); continue; }
+    f -= s.f0;
+    if (f < s.f1) { const int j = f / d.EP, k = f % d.EP; o.f1[f] = (__bf16)wat(w.w1, d.E, kU, k, j); continue; }
+    f -= s.f1;
+    if (f < s.f2) { const int j = f / kU, k = f % kU; o.f2[f] = (__bf16)wat(w.w2, kU, kU, k, j); continue; }
+    f -= s.f2;
+    if (f < s.f3) { const int j = f / kU, k = f % kU; o.f3[f] = (__bf16)wat(w.w3, kU, 3, k, j); continue; }
+    f -= s.f3;
+    // backward tables T[j][k] = W[j][k] (j = input unit, k = output unit)
+    if (f < s.g0) { const int j = f / 32, k = f % 32; o.g0[f] = (__bf16)wat(w.w0, d.F, d.S, j, k); continue; }
+    f -= s.g0;
+    if (f < s.g1) { const int j = f / kU, k = f % kU; o.g1[f] = (__bf16)wat(w.w1, d.E, kU, j, k); continue; }
+    f -= s.g1;
+    if (f < s.g2) { const int j = f / kU, k = f % kU; o.g2[f] = (__bf16)wat(w.w2, kU, kU, j, k); continue; }
+    f -= s.g2;
+    { const int j = f / 32, k = f % 32; o.g3[f] = (__bf16)wat(w.w3, kU, 3, j, k); }
   }
 }
 
@@ -365,8 +502,14 @@ __device__ __forceinline__ void add_tile(float* __restrict__ part, int64_t off, 
   }
 }
 
-template <int TM, int NW>
-__global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt, GradPack gp,
+template <int TM>
+inline size_t bwd_lds_floats(const MlpDims& d) {
+  return (size_t)TM * (d.ldF + d.ldE + d.ldS + 2 * d.ldU + (d.ncam > 0 ? d.ldU : 0) + d.ld3 + 4 +
+                       4);
+}
+
+template <int TM, int NW, bool BF>
+__global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack pk, GradPack gp,
                                                      const float* __restrict__ feats,
                                                      const float* __restrict__ dirs,
                                                      const uint32_t* __restrict__ cams,
@@ -385,7 +528,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
   float* H2p = d.ncam > 0 ? after_h2 : H2;  // pre-FiLM relu(z2) (own region with FiLM)
   float* Z3 = d.ncam > 0 ? after_h2 + TM * d.ldU : after_h2;  // TM x ld3: rgb, then dZ3
   float* Ds = Z3 + TM * d.ld3;
-  int* Cs = reinterpret_cast<int*>(Ds + TM * 3);
+  int* Cs = reinterpret_cast<int*>(Ds + TM * 4);
+  const Operands<BF> op{d, w, pk};
 
   const int wave = threadIdx.x >> 6;
   const int lane = lane_id();
@@ -398,11 +542,11 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
     __syncthreads();
     load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
     __syncthreads();
-    forward_hidden<TM, NW>(d, w, Fs, X, H0, H1, H2p, H2, Ds, Cs);
-    forward_out<TM, NW>(d, w, H2, Z3);
+    forward_hidden<TM, NW, BF>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
+    forward_out<TM, NW, BF>(d, w, op, H2, Z3);
     // dZ3 = grgb * s * (1 - s), zero outside the 3 colours and the valid rows
-    for (int e = threadIdx.x; e < TM * 16; e += blockDim.x) {
-      const int i = e / 16, c = e % 16;
+    for (int e = threadIdx.x; e < TM * d.Z3W; e += blockDim.x) {
+      const int i = e / d.Z3W, c = e % d.Z3W;
       const int64_t row = row0 + i;
       float v = 0.0f;
       if (c < 3 && row < M) {
@@ -414,18 +558,19 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
     __syncthreads();
     // dW3 (U x 3) += H2^T dZ3 ; db3
     for (int t = wave; t < kU / 16; t += NW)
-      add_tile(part, gp.w3, kU, 3, 16 * t, 0, mma_wgrad_f32(H2, d.ldU, 16 * t, Z3, d.ld3, 0, TM));
+      add_tile(part, gp.w3, kU, 3, 16 * t, 0,
+               mma_wgrad<BF>(H2, d.ldU, 16 * t, Z3, d.ld3, 0, TM));
     if (threadIdx.x < 3) {
       float s = 0.f;
       for (int i = 0; i < TM; ++i) s += Z3[i * d.ld3 + threadIdx.x];
       part[gp.b3 + threadIdx.x] += s;
     }
     __syncthreads();
-    // dH2 = dZ3 @ W3^T (K = 3 padded to 4); FiLM backward; dZ2 = dH2 * (h2p > 0)
+    // dH2 = dZ3 @ W3^T; FiLM backward; dZ2 = dH2 * (h2 > 0) written over H2
     for (int ct = wave; ct < kU / 16; ct += NW) {
       f32x4 acc[RT];
       zero_acc(acc);
-      mma_strip_f32<RT>(Z3, d.ld3, 4, GlobalB{wt.w3t, 3, kU}, 16 * ct, acc);
+      mma_strip<RT>(Z3, d.ld3, op.k3(), op.g3(), 16 * ct, acc);
       const int col = 16 * ct + (lane & 15);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
@@ -441,7 +586,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
             unsafeAtomicAdd(gemb + col, g);
             g = g * emb[col - kU / 2];
           }
-          H2[row * d.ldU + col] = hp > 0.0f ? g : 0.0f;  // dZ2 (H2 is dead after dW3)
+          H2[row * d.ldU + col] = hp > 0.0f ? g : 0.0f;
         }
     }
     __syncthreads();
@@ -449,7 +594,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
     for (int t = wave; t < (kU / 16) * (kU / 16); t += NW) {
       const int ti = t / (kU / 16), tj = t % (kU / 16);
       add_tile(part, gp.w2, kU, kU, 16 * ti, 16 * tj,
-               mma_wgrad_f32(H1, d.ldU, 16 * ti, H2, d.ldU, 16 * tj, TM));
+               mma_wgrad<BF>(H1, d.ldU, 16 * ti, H2, d.ldU, 16 * tj, TM));
     }
     for (int c = threadIdx.x; c < kU; c += blockDim.x) {
       float s = 0.f;
@@ -466,7 +611,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
         if (ct < kU / 16) {
           f32x4 acc[RT];
           zero_acc(acc);
-          mma_strip_f32<RT>(H2, d.ldU, kU, GlobalB{wt.w2t, kU, kU}, 16 * ct, acc);
+          mma_strip<RT>(H2, d.ldU, kU, op.g2(), 16 * ct, acc);
           const int col = 16 * ct + (lane & 15);
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt)
@@ -484,7 +629,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
     for (int t = wave; t < (d.EP / 16) * (kU / 16); t += NW) {
       const int ti = t / (kU / 16), tj = t % (kU / 16);
       add_tile(part, gp.w1, d.E, kU, 16 * ti, 16 * tj,
-               mma_wgrad_f32(X, d.ldE, 16 * ti, H1, d.ldU, 16 * tj, TM));
+               mma_wgrad<BF>(X, d.ldE, 16 * ti, H1, d.ldU, 16 * tj, TM));
     }
     for (int c = threadIdx.x; c < kU; c += blockDim.x) {
       float s = 0.f;
@@ -502,7 +647,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
         if (!needed) continue;
         f32x4 acc[RT];
         zero_acc(acc);
-        mma_strip_f32<RT>(H1, d.ldU, kU, GlobalB{wt.w1t, kU, d.E}, c0, acc);
+        mma_strip<RT>(H1, d.ldU, kU, op.g1(), c0, acc);
         const int col = c0 + (lane & 15);
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -539,14 +684,14 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
     for (int t = wave; t < (d.FP / 16) * 2; t += NW) {
       const int ti = t / 2, tj = t % 2;
       add_tile(part, gp.w0, d.F, d.S, 16 * ti, 16 * tj,
-               mma_wgrad_f32(Fs, d.ldF, 16 * ti, H0, d.ldS, 16 * tj, TM));
+               mma_wgrad<BF>(Fs, d.ldF, 16 * ti, H0, d.ldS, 16 * tj, TM));
     }
     // dF = dH0 @ W0^T -> global
     if (gfeat) {
       for (int ct = wave; ct < d.FP / 16; ct += NW) {
         f32x4 acc[RT];
         zero_acc(acc);
-        mma_strip_f32<RT>(H0, d.ldS, kSP, GlobalB{wt.w0t, d.S, d.F}, 16 * ct, acc);
+        mma_strip<RT>(H0, d.ldS, kSP, op.g0(), 16 * ct, acc);
         const int col = 16 * ct + (lane & 15);
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -561,23 +706,16 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpWT wt
   }
 }
 
-template <int TM>
-inline size_t bwd_lds_bytes(const MlpDims& d) {
-  size_t f = (size_t)TM * (d.ldF + d.ldE + d.ldS + 2 * d.ldU + (d.ncam > 0 ? d.ldU : 0) + d.ld3 +
-                           3 + 1);
-  return f * sizeof(float);
-}
-
 // grads[e] += sum_b partials[b][e], mapped onto the separate gradient arrays.
 __global__ void k_mlp_reduce(const float* __restrict__ partials, int64_t nblk, GradPack gp,
                              tfh_mlp_grad g) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < gp.total;
        e += (int64_t)gridDim.x * blockDim.x) {
+    if (e >= gp.b3 + 3) continue;
     float s = 0.0f;
     for (int64_t b = 0; b < nblk; ++b) s += partials[b * gp.total + e];
-    float* dst = nullptr;
-    int64_t o = 0;
-    if (e >= gp.b3 + 3) continue;
+    float* dst;
+    int64_t o;
     if (e >= gp.b3) { dst = g.b3; o = e - gp.b3; }
     else if (e >= gp.w3) { dst = g.w3; o = e - gp.w3; }
     else if (e >= gp.b2) { dst = g.b2; o = e - gp.b2; }
@@ -604,7 +742,7 @@ static int num_cus() {
 }
 
 constexpr int kFwdTM = 32, kFwdNW = 4;
-constexpr int kBwdTM = 16, kBwdNW = 8;
+constexpr int kBwdTM = 32, kBwdNW = 8;
 
 int32_t check_mlp(const tfh_mlp* p, const char* who) {
   TFH_REQUIRE(p != nullptr, "%s: null tfh_mlp", who);
@@ -621,7 +759,7 @@ int32_t check_mlp(const tfh_mlp* p, const char* who) {
               "%s: Fourier frequency counts must be in [0, 8]", who);
   TFH_REQUIRE(p->embed == nullptr || p->num_cameras >= 1, "%s: embed given with num_cameras=%lld",
               who, (long long)p->num_cameras);
-  MlpDims d = make_dims(p);
+  MlpDims d = make_dims(p, true);
   TFH_REQUIRE(d.EP <= 448, "%s: encoded width %d too large", who, d.E);
   return TFH_OK;
 }
@@ -630,55 +768,131 @@ static MlpW to_w(const tfh_mlp* p) {
   return MlpW{p->w0, p->w1, p->b1, p->w2, p->b2, p->w3, p->b3, p->embed};
 }
 
+// Workspace of the weight tables of one precision.
+static size_t pack_bytes(const MlpDims& d, bool bf) {
+  PackSizes s = pack_sizes(d);
+  if (!bf) return align_up((size_t)(s.t0 + s.t1 + s.t2 + s.t3) * sizeof(float), 256);
+  size_t n = 0;
+  for (int64_t v : {s.f0, s.f1, s.f2, s.f3, s.g0, s.g1, s.g2, s.g3})
+    n += align_up((size_t)v * sizeof(__bf16), 256);
+  return n;
+}
+
+// Carves and fills the weight tables; `need_fp32_t` selects the FP32 transposes.
+static int32_t build_pack(const tfh_mlp* p, const MlpDims& d, bool bf, bool need_fp32_t,
+                          Carve& cv, MlpPack& pk, hipStream_t st) {
+  pk = MlpPack{};
+  PackSizes s = pack_sizes(d);
+  PackOut o{};
+  if (bf) {
+    o.f0 = cv.take<__bf16>(s.f0); o.f1 = cv.take<__bf16>(s.f1);
+    o.f2 = cv.take<__bf16>(s.f2); o.f3 = cv.take<__bf16>(s.f3);
+    o.g0 = cv.take<__bf16>(s.g0); o.g1 = cv.take<__bf16>(s.g1);
+    o.g2 = cv.take<__bf16>(s.g2); o.g3 = cv.take<__bf16>(s.g3);
+  } else if (need_fp32_t) {
+    o.w0t = cv.take<float>(s.t0 + s.t1 + s.t2 + s.t3);
+    if (o.w0t) {
+      o.w1t = o.w0t + s.t0;
+      o.w2t = o.w1t + s.t1;
+      o.w3t = o.w2t + s.t2;
+    }
+  } else {
+    return TFH_OK;
+  }
+  if (!cv.ok) return fail(TFH_ERR_WORKSPACE, "mlp: workspace too small for the weight tables");
+  ProfScope ps("mlp_pack", st);
+  if (bf)
+    hipLaunchKernelGGL(k_pack_weights<true>, dim3(128), dim3(256), 0, st, d, to_w(p), o, s);
+  else
+    hipLaunchKernelGGL(k_pack_weights<false>, dim3(64), dim3(256), 0, st, d, to_w(p), o, s);
+  TFH_TRY(check_launch("mlp: weight tables"));
+  pk.w0t = o.w0t; pk.w1t = o.w1t; pk.w2t = o.w2t; pk.w3t = o.w3t;
+  pk.f0 = o.f0; pk.f1 = o.f1; pk.f2 = o.f2; pk.f3 = o.f3;
+  pk.g0 = o.g0; pk.g1 = o.g1; pk.g2 = o.g2; pk.g3 = o.g3;
+  return TFH_OK;
+}
+
+// Dynamic LDS above 64 KB needs the per-kernel opt-in.
+template <class K>
+static void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipGetLastError();
+  }
+}
+
+size_t mlp_fwd_workspace(const tfh_mlp* p, int32_t precision) {
+  const bool bf = precision == TFH_PREC_BF16;
+  return bf ? pack_bytes(make_dims(p, true), true) : 0;
+}
+
 int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
-                     int64_t M, int64_t rpr, float* rgb, int32_t precision, hipStream_t st) {
-  (void)precision;
-  MlpDims d = make_dims(p);
-  const size_t lds = fwd_lds_bytes<kFwdTM>(d);
+                     int64_t M, int64_t rpr, float* rgb, int32_t precision, void* ws,
+                     size_t ws_bytes, hipStream_t st) {
+  const bool bf = precision == TFH_PREC_BF16;
+  MlpDims d = make_dims(p, bf);
+  Carve cv(ws, ws_bytes);
+  MlpPack pk;
+  TFH_TRY(build_pack(p, d, bf, false, cv, pk, st));
+  const size_t lds = fwd_lds_floats<kFwdTM>(d) * sizeof(float);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp fwd: tile needs %zu bytes of LDS", lds);
   const int64_t blocks = ceil_div(M, kFwdTM);
   ProfScope ps("mlp_fwd_kernel", st);
-  hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW>), dim3((unsigned)blocks), dim3(kFwdNW * 64), lds,
-                     st, d, to_w(p), feats, dirs, cams, M, rpr, rgb);
+  if (bf) {
+    allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, true>, lds);
+    hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW, true>), dim3((unsigned)blocks),
+                       dim3(kFwdNW * 64), lds, st, d, to_w(p), pk, feats, dirs, cams, M, rpr, rgb);
+  } else {
+    allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, false>, lds);
+    hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW, false>), dim3((unsigned)blocks),
+                       dim3(kFwdNW * 64), lds, st, d, to_w(p), pk, feats, dirs, cams, M, rpr, rgb);
+  }
   return check_launch("mlp fwd");
 }
 
-size_t mlp_bwd_workspace(const tfh_mlp* p, int64_t M) {
-  MlpDims d = make_dims(p);
-  GradPack gp = make_pack(d);
+static int64_t bwd_blocks(int64_t M) {
   int64_t nblk = std::min<int64_t>(ceil_div(M, kBwdTM), (int64_t)num_cus());
-  if (nblk < 1) nblk = 1;
-  size_t wt = (size_t)(d.F * d.S + d.E * kU + kU * kU + kU * 3);
-  return align_up(wt * sizeof(float), 256) + align_up((size_t)nblk * gp.total * sizeof(float), 256);
+  return nblk < 1 ? 1 : nblk;
+}
+
+size_t mlp_bwd_workspace(const tfh_mlp* p, int64_t M, int32_t precision) {
+  const bool bf = precision == TFH_PREC_BF16;
+  MlpDims d = make_dims(p, bf);
+  GradPack gp = make_pack(d);
+  return pack_bytes(d, bf) + align_up((size_t)bwd_blocks(M) * gp.total * sizeof(float), 256);
 }
 
 int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
-                     int64_t M, int64_t rpr, const float* grgb, float* gfeat, const tfh_mlp_grad* g,
-                     int32_t precision, void* ws, size_t ws_bytes, hipStream_t st) {
-  (void)precision;
-  MlpDims d = make_dims(p);
+                     int64_t M, int64_t rpr, const float* grgb, float* gfeat,
+                     const tfh_mlp_grad* g, int32_t precision, void* ws, size_t ws_bytes,
+                     hipStream_t st) {
+  const bool bf = precision == TFH_PREC_BF16;
+  MlpDims d = make_dims(p, bf);
   GradPack gp = make_pack(d);
-  int64_t nblk = std::min<int64_t>(ceil_div(M, kBwdTM), (int64_t)num_cus());
-  if (nblk < 1) nblk = 1;
+  const int64_t nblk = bwd_blocks(M);
   Carve cv(ws, ws_bytes);
-  MlpWT wt;
-  wt.w0t = cv.take<float>((int64_t)d.F * d.S + (int64_t)d.E * kU + kU * kU + kU * 3);
+  MlpPack pk;
+  TFH_TRY(build_pack(p, d, bf, true, cv, pk, st));
   float* partials = cv.take<float>(nblk * gp.total);
   if (!cv.ok)
     return fail(TFH_ERR_WORKSPACE, "mlp bwd: workspace of %zu bytes, need %zu", ws_bytes,
-                mlp_bwd_workspace(p, M));
-  wt.w1t = wt.w0t + (int64_t)d.F * d.S;
-  wt.w2t = wt.w1t + (int64_t)d.E * kU;
-  wt.w3t = wt.w2t + kU * kU;
-  hipLaunchKernelGGL(k_transpose_weights, dim3(64), dim3(256), 0, st, d, to_w(p), wt);
-  TFH_TRY(check_launch("mlp bwd: weight transpose"));
-  const size_t lds = bwd_lds_bytes<kBwdTM>(d);
+                mlp_bwd_workspace(p, M, precision));
+  const size_t lds = bwd_lds_floats<kBwdTM>(d) * sizeof(float);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp bwd: tile needs %zu bytes of LDS", lds);
   {
     ProfScope ps("mlp_bwd", st);
-    hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW>), dim3((unsigned)nblk), dim3(kBwdNW * 64), lds,
-                       st, d, to_w(p), wt, gp, feats, dirs, cams, grgb, M, rpr, gfeat, partials,
-                       g->embed);
+    if (bf) {
+      allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, true>, lds);
+      hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, true>), dim3((unsigned)nblk),
+                         dim3(kBwdNW * 64), lds, st, d, to_w(p), pk, gp, feats, dirs, cams, grgb,
+                         M, rpr, gfeat, partials, g->embed);
+    } else {
+      allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, false>, lds);
+      hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, false>), dim3((unsigned)nblk),
+                         dim3(kBwdNW * 64), lds, st, d, to_w(p), pk, gp, feats, dirs, cams, grgb,
+                         M, rpr, gfeat, partials, g->embed);
+    }
   }
   TFH_TRY(check_launch("mlp bwd"));
   ProfScope ps("mlp_reduce", st);
@@ -693,9 +907,15 @@ using namespace tfh;
 
 extern "C" {
 
+size_t tfh_feature_mlp_fwd_workspace_size(const tfh_mlp* mlp, int32_t precision) {
+  if (check_mlp(mlp, "tfh_feature_mlp_fwd_workspace_size") != TFH_OK) return 0;
+  return mlp_fwd_workspace(mlp, precision);
+}
+
 int32_t tfh_feature_mlp_fwd(const tfh_mlp* mlp, const float* features, const float* viewdirs,
                             const uint32_t* camera_indices, int64_t num_rows, float* rgb,
-                            int32_t precision, tfh_stream_t stream) {
+                            int32_t precision, void* workspace, size_t workspace_bytes,
+                            tfh_stream_t stream) {
   TFH_TRY(check_mlp(mlp, "tfh_feature_mlp_fwd"));
   TFH_REQUIRE(num_rows >= 0, "tfh_feature_mlp_fwd: negative num_rows");
   TFH_REQUIRE(precision == TFH_PREC_FP32 || precision == TFH_PREC_BF16,
@@ -706,12 +926,13 @@ int32_t tfh_feature_mlp_fwd(const tfh_mlp* mlp, const float* features, const flo
               "tfh_feature_mlp_fwd: camera_indices required with an embedding");
   TFH_TRY(require_device());
   return mlp_fwd_rows(mlp, features, viewdirs, camera_indices, num_rows, 1, rgb, precision,
-                      as_stream(stream));
+                      workspace, workspace_bytes, as_stream(stream));
 }
 
-size_t tfh_feature_mlp_bwd_workspace_size(const tfh_mlp* mlp, int64_t num_rows) {
+size_t tfh_feature_mlp_bwd_workspace_size(const tfh_mlp* mlp, int64_t num_rows,
+                                          int32_t precision) {
   if (check_mlp(mlp, "tfh_feature_mlp_bwd_workspace_size") != TFH_OK || num_rows < 0) return 0;
-  return mlp_bwd_workspace(mlp, num_rows);
+  return mlp_bwd_workspace(mlp, num_rows, precision);
 }
 
 int32_t tfh_feature_mlp_bwd(const tfh_mlp* mlp, const float* features, const float* viewdirs,

@@ -22,9 +22,9 @@ namespace tfh {
 
 int32_t check_mlp(const tfh_mlp* p, const char* who);
 int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
-                     int64_t M, int64_t rows_per_ray, float* rgb, int32_t precision,
-                     hipStream_t st);
-size_t mlp_bwd_workspace(const tfh_mlp* p, int64_t M);
+                     int64_t M, int64_t rows_per_ray, float* rgb, int32_t precision, void* ws,
+                     size_t ws_bytes, hipStream_t st);
+size_t mlp_bwd_workspace(const tfh_mlp* p, int64_t M, int32_t precision);
 int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
                      int64_t M, int64_t rows_per_ray, const float* grgb, float* gfeat,
                      const tfh_mlp_grad* g, int32_t precision, void* ws, size_t ws_bytes,
@@ -600,7 +600,9 @@ static size_t carve_render(Carve& cv, const tfh_render_config* cfg, const tfh_ml
   ws.gout = cv.take<float>(R * 3);
   ws.dc = cv.take<float>(M * 3);
   ws.dfeat = cv.take<float>(M * F);
-  ws.mlp_ws_bytes = mlp ? mlp_bwd_workspace(mlp, M) : 0;
+  // one region serves the forward weight tables and, later, the backward's
+  // tables + partials (the backward's need is the larger)
+  ws.mlp_ws_bytes = mlp ? mlp_bwd_workspace(mlp, M, cfg->mlp_precision) : 0;
   ws.mlp_ws = cv.take<char>((int64_t)ws.mlp_ws_bytes);
   return cv.off;
 }
@@ -647,7 +649,8 @@ static int32_t render_forward(const tfh_vm* density, const tfh_vm* appearance, c
   }
   {
     ProfScope ps("mlp_fwd", st);
-    TFH_TRY(mlp_fwd_rows(mlp, ws.feats, in.d, cams, M, k, ws.rgb, cfg->mlp_precision, st));
+    TFH_TRY(mlp_fwd_rows(mlp, ws.feats, in.d, cams, M, k, ws.rgb, cfg->mlp_precision, ws.mlp_ws,
+                         ws.mlp_ws_bytes, st));
   }
   ProfScope ps("composite_rgb", st);
   hipLaunchKernelGGL(k_composite_rgb, dim3((unsigned)R), dim3(64), 0, st, ws.d.w, ws.d.idx, ws.rgb,

@@ -82,8 +82,9 @@ _SIGS = {
                                       c_p]),
     "tfh_segment_probabilities": (c_i32, [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p]),
     "tfh_topk_select": (c_i32, [c_p, c_i64, c_i64, c_i64, c_p, c_p, c_p]),
-    "tfh_feature_mlp_fwd": (c_i32, [P(Mlp), c_p, c_p, c_p, c_i64, c_p, c_i32, c_p]),
-    "tfh_feature_mlp_bwd_workspace_size": (c_sz, [P(Mlp), c_i64]),
+    "tfh_feature_mlp_fwd_workspace_size": (c_sz, [P(Mlp), c_i32]),
+    "tfh_feature_mlp_fwd": (c_i32, [P(Mlp), c_p, c_p, c_p, c_i64, c_p, c_i32, c_p, c_sz, c_p]),
+    "tfh_feature_mlp_bwd_workspace_size": (c_sz, [P(Mlp), c_i64, c_i32]),
     "tfh_feature_mlp_bwd": (c_i32, [P(Mlp), c_p, c_p, c_p, c_i64, c_p, c_p, P(MlpGrad), c_i32,
                                     c_p, c_sz, c_p]),
     "tfh_render_workspace_size": (c_sz, [P(RenderConfig), P(Mlp), c_i64, c_i64]),

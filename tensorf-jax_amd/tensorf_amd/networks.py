@@ -122,8 +122,12 @@ class _MlpFn(torch.autograd.Function):
         m = f.shape[0]
         rgb = torch.empty((m, 3), dtype=torch.float32, device=f.device)
         st = mlp.c_struct(params, f.shape[1])
+        ws_bytes = int(_lib.lib.tfh_feature_mlp_fwd_workspace_size(ctypes.byref(st),
+                                                                   mlp.precision_code))
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=f.device)
         _lib.call("tfh_feature_mlp_fwd", ctypes.byref(st), _lib.ptr(f), _lib.ptr(d),
-                  _lib.ptr(cams), m, _lib.ptr(rgb), mlp.precision_code, _lib.stream())
+                  _lib.ptr(cams), m, _lib.ptr(rgb), mlp.precision_code, _lib.ptr(ws), ws.numel(),
+                  _lib.stream())
         ctx.mlp, ctx.params = mlp, params
         ctx.save_for_backward(f, d, cams if cams is not None else torch.empty(0))
         ctx.has_cams = cams is not None
@@ -140,7 +144,8 @@ class _MlpFn(torch.autograd.Function):
         st = mlp.c_struct(params, f.shape[1])
         g = _lib.MlpGrad(*[_lib.ptr(t) for t in grads], *([None] * (8 - len(grads))))
         m = f.shape[0]
-        ws_bytes = _lib.lib.tfh_feature_mlp_bwd_workspace_size(ctypes.byref(st), m)
+        ws_bytes = _lib.lib.tfh_feature_mlp_bwd_workspace_size(ctypes.byref(st), m,
+                                                               mlp.precision_code)
         ws = torch.empty(max(int(ws_bytes), 1), dtype=torch.uint8, device=f.device)
         grgb = grgb.contiguous()  # keep the buffer alive across the asynchronous call
         _lib.call("tfh_feature_mlp_bwd", ctypes.byref(st), _lib.ptr(f), _lib.ptr(d),

@@ -60,7 +60,8 @@ def test_vm_interpolate_fwd_bwd(lib, c, g):
     got = out.detach().cpu().numpy()
     err32 = report("features vs fp32 oracle", got, want32)
     err = report("features vs fp64 oracle", got, want)
-    assert np.all(err32 <= 1e-5 * np.abs(want32) + 1e-6 * scale)
+    # a 1-ulp difference of u (~G * 6e-8 cells) moves a value by up to ~G * 6e-8 * scale
+    assert np.all(err32 <= 1e-5 * np.abs(want32) + 1e-6 * max(1.0, g / 16) * scale)
     # vs float64 the FP32 texel coordinate u=(x+1)/2*(G-1) carries ~G*eps cells of error
     assert err.max() <= 4e-7 * max(g, 8) * scale
     gout = f32(rng.normal(size=want.shape))
