@@ -19,6 +19,9 @@
 // gradients into a per-workgroup partial buffer (plain read-modify-write,
 // each partial tile owned by one wave), reduced across workgroups by
 // k_mlp_reduce.
+#include <stdlib.h>
+#include <string.h>
+
 #include <algorithm>
 
 #include "tfh_common.h"
@@ -68,6 +71,25 @@ inline MlpDims make_dims(const tfh_mlp* p, bool bf) {
   return d;
 }
 
+// Dims of the standard appearance MLP (48 channels x 3 modes, squash 27,
+// 2+2 Fourier frequencies, no camera embedding) as compile-time constants, so
+// the kernels built for it fold every LDS offset and tile count.
+template <bool BF>
+__host__ __device__ constexpr MlpDims std_dims() {
+  const int kp = BF ? 32 : 16, pad = BF ? 4 : 2;
+  const int F = 144, S = 27, nf = 2, nv = 2;
+  const int E = S + 3 + 2 * nf * S + 2 * nv * 3;
+  const int FP = (F + kp - 1) / kp * kp, EP = (E + kp - 1) / kp * kp;
+  const int Z3W = BF ? 32 : 16;
+  return MlpDims{F, FP, S, E, EP, nf, nv, 0, Z3W, FP + pad, EP + pad, 128 + pad, 32 + pad,
+                 Z3W + pad};
+}
+
+inline bool is_std_dims(const MlpDims& d, bool bf) {
+  const MlpDims s = bf ? std_dims<true>() : std_dims<false>();
+  return d.F == s.F && d.S == s.S && d.nf == s.nf && d.nv == s.nv && d.ncam == 0;
+}
+
 struct MlpW {  // FP32 parameters (device pointers)
   const float *w0, *w1, *b1, *w2, *b2, *w3, *b3, *embed;
 };
@@ -99,69 +121,101 @@ __device__ __forceinline__ bf16x8 to_bf16x8(const float4& lo, const float4& hi) 
   return v;
 }
 
+// Raw buffer resource over `bytes` bytes at `base`: loads take a 32-bit VGPR
+// byte offset (no per-load 64-bit scalar address) and read 0 past the end.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+constexpr int kOOB = 0x7ffffff0;  // a byte offset past every buffer: loads return 0
+
 // Row-major FP32 global matrix as a B operand with zero padding outside (rows, cols).
 struct GlobalB {
-  const float* w;
+  __amdgpu_buffer_rsrc_t r;
   int rows, cols;
+  __device__ __forceinline__ GlobalB(const float* w, int rows_, int cols_)
+      : r(buffer_rsrc(w, rows_ * cols_ * 4)), rows(rows_), cols(cols_) {}
   __device__ __forceinline__ float operator()(int k, int j) const {
-    return (k < rows && j < cols) ? w[k * cols + j] : 0.0f;
+    const int off = (k < rows && j < cols) ? (k * cols + j) * 4 : kOOB;
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
   }
 };
 
 // BF16 table T[j][k] (stride ld) as a B operand: B(k, j) = T[j][k].
 struct PackedB {
-  const __bf16* t;
+  __amdgpu_buffer_rsrc_t r;
   int ld;
+  __device__ __forceinline__ PackedB(const __bf16* t, int ld_, int rows)
+      : r(buffer_rsrc(t, rows * ld_ * 2)), ld(ld_) {}
   __device__ __forceinline__ bf16x8 load8(int k, int j) const {
-    return *reinterpret_cast<const bf16x8*>(t + (size_t)j * ld + k);
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    const i32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (j * ld + k) * 2, 0, 0);
+    return __builtin_bit_cast(bf16x8, v);
   }
 };
 
 // acc[rt] += A(rows 16*rt.., K) * B(K, cols col0..col0+15), rt < NR; A from
-// LDS (row-major, lda).
-template <int NR>
+// LDS (row-major, lda), K <= KMAX. The whole B column strip is loaded into
+// registers first (KMAX/4 floats or KMAX/32 bf16x8 per lane) so its L2
+// latency overlaps; A is read from LDS per k-step.
+template <int NR, int KMAX>
 __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, GlobalB b,
                                           int col0, f32x4 (&acc)[NR]) {
   const int lane = lane_id();
   const int r = lane & 15, kq = lane >> 4;
-#pragma unroll 4
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const float bv = b(k0 + kq, col0 + r);
+  constexpr int NS = KMAX / 4;
+  for (int kc = 0; kc < K; kc += KMAX) {  // one pass for K <= KMAX
+    float bb[NS];
 #pragma unroll
-    for (int rt = 0; rt < NR; ++rt) {
-      const float a = A[(16 * rt + r) * lda + k0 + kq];
-      acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc[rt], 0, 0, 0);
+    for (int i = 0; i < NS; ++i) bb[i] = (kc + 4 * i < K) ? b(kc + 4 * i + kq, col0 + r) : 0.0f;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      if (kc + 4 * i < K) {
+#pragma unroll
+        for (int rt = 0; rt < NR; ++rt) {
+          const float a = A[(16 * rt + r) * lda + kc + 4 * i + kq];
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb[i], acc[rt], 0, 0, 0);
+        }
+      }
     }
   }
 }
 
-template <int NR>
+template <int NR, int KMAX>
 __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB b,
                                           int col0, f32x4 (&acc)[NR]) {
   const int lane = lane_id();
   const int r = lane & 15, kq = lane >> 4;
-#pragma unroll 2
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    const bf16x8 bv = b.load8(k0 + 8 * kq, col0 + r);
+  constexpr int NS = (KMAX + 31) / 32;
+  for (int kc = 0; kc < K; kc += 32 * NS) {  // one pass for K <= KMAX
+    bf16x8 bb[NS];
 #pragma unroll
-    for (int rt = 0; rt < NR; ++rt) {
-      const float* ap = A + (16 * rt + r) * lda + k0 + 8 * kq;
-      const bf16x8 av = to_bf16x8(*reinterpret_cast<const float4*>(ap),
-                                  *reinterpret_cast<const float4*>(ap + 4));
-      acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[rt], 0, 0, 0);
+    for (int i = 0; i < NS; ++i)
+      if (kc + 32 * i < K) bb[i] = b.load8(kc + 32 * i + 8 * kq, col0 + r);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      if (kc + 32 * i < K) {
+#pragma unroll
+        for (int rt = 0; rt < NR; ++rt) {
+          const float* ap = A + (16 * rt + r) * lda + kc + 32 * i + 8 * kq;
+          const bf16x8 av = to_bf16x8(*reinterpret_cast<const float4*>(ap),
+                                      *reinterpret_cast<const float4*>(ap + 4));
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bb[i], acc[rt], 0, 0, 0);
+        }
+      }
     }
   }
 }
 
 // Weight-gradient tile: acc(i, j) = sum_k X[k][row0 + i] * Y[k][col0 + j],
 // k < K (the rows of the tile).
-template <bool BF>
-__device__ __forceinline__ f32x4 mma_wgrad(const float* __restrict__ X, int ldx, int row0,
-                                           const float* __restrict__ Y, int ldy, int col0, int K) {
+template <bool BF, int K>
+__device__ __forceinline__ void mma_wgrad_acc(const float* __restrict__ X, int ldx, int row0,
+                                              const float* __restrict__ Y, int ldy, int col0,
+                                              f32x4& acc) {
   const int lane = lane_id();
   const int r = lane & 15, kq = lane >> 4;
-  f32x4 acc{0.f, 0.f, 0.f, 0.f};
   if constexpr (BF) {
+#pragma unroll
     for (int k0 = 0; k0 < K; k0 += 32) {
       bf16x8 av, bv;
 #pragma unroll
@@ -172,13 +226,20 @@ __device__ __forceinline__ f32x4 mma_wgrad(const float* __restrict__ X, int ldx,
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
     }
   } else {
-#pragma unroll 4
+#pragma unroll
     for (int k0 = 0; k0 < K; k0 += 4) {
       const float a = X[(k0 + kq) * ldx + row0 + r];
       const float b = Y[(k0 + kq) * ldy + col0 + r];
       acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
     }
   }
+}
+
+template <bool BF, int K>
+__device__ __forceinline__ f32x4 mma_wgrad(const float* __restrict__ X, int ldx, int row0,
+                                           const float* __restrict__ Y, int ldy, int col0) {
+  f32x4 acc{0.f, 0.f, 0.f, 0.f};
+  mma_wgrad_acc<BF, K>(X, ldx, row0, Y, ldy, col0, acc);
   return acc;
 }
 
@@ -191,14 +252,14 @@ struct Operands<false> {
   const MlpDims& d;
   const MlpW& w;
   const MlpPack& p;
-  __device__ GlobalB l0() const { return GlobalB{w.w0, d.F, d.S}; }
-  __device__ GlobalB l1() const { return GlobalB{w.w1, d.E, kU}; }
-  __device__ GlobalB l2() const { return GlobalB{w.w2, kU, kU}; }
-  __device__ GlobalB l3() const { return GlobalB{w.w3, kU, 3}; }
-  __device__ GlobalB g3() const { return GlobalB{p.w3t, 3, kU}; }
-  __device__ GlobalB g2() const { return GlobalB{p.w2t, kU, kU}; }
-  __device__ GlobalB g1() const { return GlobalB{p.w1t, kU, d.E}; }
-  __device__ GlobalB g0() const { return GlobalB{p.w0t, d.S, d.F}; }
+  __device__ GlobalB l0() const { return GlobalB(w.w0, d.F, d.S); }
+  __device__ GlobalB l1() const { return GlobalB(w.w1, d.E, kU); }
+  __device__ GlobalB l2() const { return GlobalB(w.w2, kU, kU); }
+  __device__ GlobalB l3() const { return GlobalB(w.w3, kU, 3); }
+  __device__ GlobalB g3() const { return GlobalB(p.w3t, 3, kU); }
+  __device__ GlobalB g2() const { return GlobalB(p.w2t, kU, kU); }
+  __device__ GlobalB g1() const { return GlobalB(p.w1t, kU, d.E); }
+  __device__ GlobalB g0() const { return GlobalB(p.w0t, d.S, d.F); }
   __device__ int k0() const { return round_up(d.F, 4); }
   __device__ int k3() const { return 4; }  // K of dZ3 @ W3^T (3 padded to 4)
 };
@@ -208,14 +269,14 @@ struct Operands<true> {
   const MlpDims& d;
   const MlpW& w;
   const MlpPack& p;
-  __device__ PackedB l0() const { return PackedB{p.f0, d.FP}; }
-  __device__ PackedB l1() const { return PackedB{p.f1, d.EP}; }
-  __device__ PackedB l2() const { return PackedB{p.f2, kU}; }
-  __device__ PackedB l3() const { return PackedB{p.f3, kU}; }
-  __device__ PackedB g3() const { return PackedB{p.g3, 32}; }
-  __device__ PackedB g2() const { return PackedB{p.g2, kU}; }
-  __device__ PackedB g1() const { return PackedB{p.g1, kU}; }
-  __device__ PackedB g0() const { return PackedB{p.g0, 32}; }
+  __device__ PackedB l0() const { return PackedB(p.f0, d.FP, 32); }
+  __device__ PackedB l1() const { return PackedB(p.f1, d.EP, kU); }
+  __device__ PackedB l2() const { return PackedB(p.f2, kU, kU); }
+  __device__ PackedB l3() const { return PackedB(p.f3, kU, 16); }
+  __device__ PackedB g3() const { return PackedB(p.g3, 32, kU); }
+  __device__ PackedB g2() const { return PackedB(p.g2, kU, kU); }
+  __device__ PackedB g1() const { return PackedB(p.g1, kU, d.EP); }
+  __device__ PackedB g0() const { return PackedB(p.g0, 32, d.FP); }
   __device__ int k0() const { return d.FP; }
   __device__ int k3() const { return 32; }
 };
@@ -291,7 +352,7 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
   for (int ct = wave; ct < 2; ct += NW) {
     f32x4 acc[RT];
     zero_acc(acc);
-    mma_strip<RT>(Fs, d.ldF, op.k0(), op.l0(), 16 * ct, acc);
+    mma_strip<RT, 192>(Fs, d.ldF, op.k0(), op.l0(), 16 * ct, acc);
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -310,7 +371,7 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
   for (int ct = wave; ct < kU / 16; ct += NW) {
     f32x4 acc[RT];
     zero_acc(acc);
-    mma_strip<RT>(X, d.ldE, d.EP, op.l1(), 16 * ct, acc);
+    mma_strip<RT, 160>(X, d.ldE, d.EP, op.l1(), 16 * ct, acc);
     const int col = 16 * ct + (lane & 15);
     const float bias = w.b1[col];
 #pragma unroll
@@ -326,7 +387,7 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
   for (int ct = wave; ct < kU / 16; ct += NW) {
     f32x4 acc[RT];
     zero_acc(acc);
-    mma_strip<RT>(H1, d.ldU, kU, op.l2(), 16 * ct, acc);
+    mma_strip<RT, kU>(H1, d.ldU, kU, op.l2(), 16 * ct, acc);
     const int col = 16 * ct + (lane & 15);
     const float bias = w.b2[col];
 #pragma unroll
@@ -357,7 +418,7 @@ __device__ void forward_out(const MlpDims& d, const MlpW& w, const Operands<BF>&
   for (int rt = wave; rt < RT; rt += NW) {
     f32x4 acc[1];
     zero_acc(acc);
-    mma_strip<1>(H2 + 16 * rt * d.ldU, d.ldU, kU, op.l3(), 0, acc);
+    mma_strip<1, kU>(H2 + 16 * rt * d.ldU, d.ldU, kU, op.l3(), 0, acc);
     const int col = lane & 15;
     if (col < 3) {
       const float bias = w.b3[col];
@@ -378,13 +439,14 @@ inline size_t fwd_lds_floats(const MlpDims& d) {
   return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + d.ld3 + 4);
 }
 
-template <int TM, int NW, bool BF>
-__global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d, MlpW w, MlpPack pk,
+template <int TM, int NW, bool BF, bool STD>
+__global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpPack pk,
                                                      const float* __restrict__ feats,
                                                      const float* __restrict__ dirs,
                                                      const uint32_t* __restrict__ cams,
                                                      int64_t M, int64_t rpr,
                                                      float* __restrict__ rgb) {
+  const MlpDims d = STD ? std_dims<BF>() : d_arg;
   extern __shared__ float smem[];
   // regions: A = Fs, later H1; B = X, later H2; small: Ds, R3, Cs
   float* regA = smem;
@@ -413,8 +475,8 @@ struct GradPack {
   int64_t w0, w1, b1, w2, b2, w3, b3, total;
 };
 
-inline GradPack make_pack(const MlpDims& d) {
-  GradPack p;
+__host__ __device__ constexpr GradPack make_pack(const MlpDims& d) {
+  GradPack p{};
   int64_t o = 0;
   p.w0 = o; o += (int64_t)d.F * d.S;
   p.w1 = o; o += (int64_t)d.E * kU;
@@ -423,7 +485,7 @@ inline GradPack make_pack(const MlpDims& d) {
   p.b2 = o; o += kU;
   p.w3 = o; o += (int64_t)kU * 3;
   p.b3 = o; o += 3;
-  p.total = round_up((int)o, 64);
+  p.total = (o + 63) / 64 * 64;
   return p;
 }
 
@@ -508,8 +570,83 @@ inline size_t bwd_lds_floats(const MlpDims& d) {
                        4);
 }
 
-template <int TM, int NW, bool BF>
-__global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack pk, GradPack gp,
+// Weight-gradient tiles (16x16) of the four layers, numbered
+// [dW3 | dW2 | dW1 | dW0]. With NT > 0 wave w accumulates tiles
+// w, w+NW, ..., w+(NT-1)NW in registers over all its row tiles and stores
+// them once at the end; with NT == 0 every row tile read-modify-writes the
+// workgroup partial instead (used when the tiles do not fit NT*NW).
+struct WTiles {
+  int n3, n2, n1, n0, b2, b1, b0, total;
+};
+
+__host__ __device__ inline WTiles wtiles(const MlpDims& d) {
+  WTiles t;
+  t.n3 = kU / 16;
+  t.n2 = (kU / 16) * (kU / 16);
+  t.n1 = (d.EP / 16) * (kU / 16);
+  t.n0 = (d.FP / 16) * 2;
+  t.b2 = t.n3;
+  t.b1 = t.b2 + t.n2;
+  t.b0 = t.b1 + t.n1;
+  t.total = t.b0 + t.n0;
+  return t;
+}
+
+// Stores a 16x16 tile (D layout) into the (rows x cols) block at `off`.
+__device__ __forceinline__ void store_tile(float* __restrict__ part, int64_t off, int rows,
+                                           int cols, int row0, int col0, const f32x4& acc) {
+  const int lane = lane_id();
+  const int col = col0 + (lane & 15);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int row = row0 + 4 * (lane >> 4) + e;
+    if (row < rows && col < cols) part[off + (int64_t)row * cols + col] = acc[e];
+  }
+}
+
+// Weight-gradient phase of one layer: for every tile of the layer (n tiles,
+// tcols column tiles per row of tiles) add X^T Y over the TM rows of the row
+// tile. With NT > 0, wave w owns the layer's tiles w, w+NW, ..., held in the
+// register slots [SLOT0, SLOT0 + CNT); otherwise tiles read-modify-write the
+// workgroup partial.
+template <int TM, int NW, bool BF, int NT, int SLOT0, int CNT>
+__device__ __forceinline__ void wgrad_phase(f32x4 (&wacc)[NT > 0 ? NT : 1], int wave, int n,
+                                            int tcols, const float* Xm, int ldx,
+                                            const float* Ym, int ldy, float* part, int64_t off,
+                                            int rows, int cols) {
+  if constexpr (NT > 0) {
+#pragma unroll
+    for (int si = 0; si < CNT; ++si) {
+      const int t = wave + NW * si;
+      if (t < n) {
+        const int ti = t / tcols, tj = t % tcols;
+        mma_wgrad_acc<BF, TM>(Xm, ldx, 16 * ti, Ym, ldy, 16 * tj, wacc[SLOT0 + si]);
+      }
+      // keep the operand loads of later tiles from being hoisted (they would
+      // all be live at once)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+    for (int t = wave; t < n; t += NW) {
+      const int ti = t / tcols, tj = t % tcols;
+      add_tile(part, off, rows, cols, 16 * ti, 16 * tj,
+               mma_wgrad<BF, TM>(Xm, ldx, 16 * ti, Ym, ldy, 16 * tj));
+    }
+  }
+}
+
+// Register slots of the weight-gradient tiles per wave (kBwdNW waves): dW3 8
+// tiles, dW2 64, dW1 up to 80 (encoded width <= 160), dW0 up to 20
+// (features <= 160).
+constexpr int kBwdNW = 8;
+constexpr int kCnt3 = (8 + kBwdNW - 1) / kBwdNW, kSlot3 = 0;
+constexpr int kCnt2 = (64 + kBwdNW - 1) / kBwdNW, kSlot2 = kSlot3 + kCnt3;
+constexpr int kCnt1 = (80 + kBwdNW - 1) / kBwdNW, kSlot1 = kSlot2 + kCnt2;
+constexpr int kCnt0 = (20 + kBwdNW - 1) / kBwdNW, kSlot0 = kSlot1 + kCnt1;
+constexpr int kSlots = kSlot0 + kCnt0;
+
+template <int TM, int NW, bool BF, int NT, bool STD>
+__global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpPack pk, GradPack gp_arg,
                                                      const float* __restrict__ feats,
                                                      const float* __restrict__ dirs,
                                                      const uint32_t* __restrict__ cams,
@@ -518,6 +655,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack 
                                                      float* __restrict__ partials,
                                                      float* __restrict__ gembed) {
   constexpr int RT = TM / 16;
+  const MlpDims d = STD ? std_dims<BF>() : d_arg;
+  const GradPack gp = STD ? make_pack(std_dims<BF>()) : gp_arg;
   extern __shared__ float smem[];
   float* Fs = smem;                         // TM x ldF
   float* X = Fs + TM * d.ldF;               // TM x ldE   (later dX)
@@ -534,7 +673,13 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack 
   const int wave = threadIdx.x >> 6;
   const int lane = lane_id();
   float* part = partials + (int64_t)blockIdx.x * gp.total;
-  for (int64_t e = threadIdx.x; e < gp.total; e += blockDim.x) part[e] = 0.0f;
+  const WTiles wt = wtiles(d);
+  f32x4 wacc[NT > 0 ? NT : 1];
+#pragma unroll
+  for (int i = 0; i < (NT > 0 ? NT : 1); ++i) wacc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float db1 = 0.f, db2 = 0.f, db3 = 0.f;  // bias gradients of column threadIdx.x
+  if (NT == 0)
+    for (int64_t e = threadIdx.x; e < gp.total; e += blockDim.x) part[e] = 0.0f;
 
   const int64_t ntiles = (M + TM - 1) / TM;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -557,20 +702,16 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack 
     }
     __syncthreads();
     // dW3 (U x 3) += H2^T dZ3 ; db3
-    for (int t = wave; t < kU / 16; t += NW)
-      add_tile(part, gp.w3, kU, 3, 16 * t, 0,
-               mma_wgrad<BF>(H2, d.ldU, 16 * t, Z3, d.ld3, 0, TM));
-    if (threadIdx.x < 3) {
-      float s = 0.f;
-      for (int i = 0; i < TM; ++i) s += Z3[i * d.ld3 + threadIdx.x];
-      part[gp.b3 + threadIdx.x] += s;
-    }
+    wgrad_phase<TM, NW, BF, NT, kSlot3, kCnt3>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3, d.ld3, part,
+                                               gp.w3, kU, 3);
+    if (threadIdx.x < 3)
+      for (int i = 0; i < TM; ++i) db3 += Z3[i * d.ld3 + threadIdx.x];
     __syncthreads();
     // dH2 = dZ3 @ W3^T; FiLM backward; dZ2 = dH2 * (h2 > 0) written over H2
     for (int ct = wave; ct < kU / 16; ct += NW) {
       f32x4 acc[RT];
       zero_acc(acc);
-      mma_strip<RT>(Z3, d.ld3, op.k3(), op.g3(), 16 * ct, acc);
+      mma_strip<RT, 32>(Z3, d.ld3, op.k3(), op.g3(), 16 * ct, acc);
       const int col = 16 * ct + (lane & 15);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
@@ -591,16 +732,10 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack 
     }
     __syncthreads();
     // dW2 (U x U) += H1^T dZ2 ; db2
-    for (int t = wave; t < (kU / 16) * (kU / 16); t += NW) {
-      const int ti = t / (kU / 16), tj = t % (kU / 16);
-      add_tile(part, gp.w2, kU, kU, 16 * ti, 16 * tj,
-               mma_wgrad<BF>(H1, d.ldU, 16 * ti, H2, d.ldU, 16 * tj, TM));
-    }
-    for (int c = threadIdx.x; c < kU; c += blockDim.x) {
-      float s = 0.f;
-      for (int i = 0; i < TM; ++i) s += H2[i * d.ldU + c];
-      part[gp.b2 + c] += s;
-    }
+    wgrad_phase<TM, NW, BF, NT, kSlot2, kCnt2>(wacc, wave, wt.n2, kU / 16, H1, d.ldU, H2,
+                                               d.ldU, part, gp.w2, kU, kU);
+    if (threadIdx.x < kU)
+      for (int i = 0; i < TM; ++i) db2 += H2[i * d.ldU + threadIdx.x];
     __syncthreads();
     // dH1 = dZ2 @ W2^T ; dZ1 = dH1 * (h1 > 0) written over H1 (dW2 is done)
     {
@@ -611,7 +746,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack 
         if (ct < kU / 16) {
           f32x4 acc[RT];
           zero_acc(acc);
-          mma_strip<RT>(H2, d.ldU, kU, op.g2(), 16 * ct, acc);
+          mma_strip<RT, kU>(H2, d.ldU, kU, op.g2(), 16 * ct, acc);
           const int col = 16 * ct + (lane & 15);
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt)
@@ -626,16 +761,10 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack 
     }
     __syncthreads();
     // dW1 (E x U) += X^T dZ1 ; db1
-    for (int t = wave; t < (d.EP / 16) * (kU / 16); t += NW) {
-      const int ti = t / (kU / 16), tj = t % (kU / 16);
-      add_tile(part, gp.w1, d.E, kU, 16 * ti, 16 * tj,
-               mma_wgrad<BF>(X, d.ldE, 16 * ti, H1, d.ldU, 16 * tj, TM));
-    }
-    for (int c = threadIdx.x; c < kU; c += blockDim.x) {
-      float s = 0.f;
-      for (int i = 0; i < TM; ++i) s += H1[i * d.ldU + c];
-      part[gp.b1 + c] += s;
-    }
+    wgrad_phase<TM, NW, BF, NT, kSlot1, kCnt1>(wacc, wave, wt.n1, kU / 16, X, d.ldE, H1,
+                                               d.ldU, part, gp.w1, d.E, kU);
+    if (threadIdx.x < kU)
+      for (int i = 0; i < TM; ++i) db1 += H1[i * d.ldU + threadIdx.x];
     __syncthreads();
     // dX = dZ1 @ W1^T for the columns that feed h0: [0, S) and the FE(h0) block
     {
@@ -647,7 +776,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack 
         if (!needed) continue;
         f32x4 acc[RT];
         zero_acc(acc);
-        mma_strip<RT>(H1, d.ldU, kU, op.g1(), c0, acc);
+        mma_strip<RT, kU>(H1, d.ldU, kU, op.g1(), c0, acc);
         const int col = c0 + (lane & 15);
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -681,17 +810,14 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack 
     }
     __syncthreads();
     // dW0 (F x S) += F^T dH0
-    for (int t = wave; t < (d.FP / 16) * 2; t += NW) {
-      const int ti = t / 2, tj = t % 2;
-      add_tile(part, gp.w0, d.F, d.S, 16 * ti, 16 * tj,
-               mma_wgrad<BF>(Fs, d.ldF, 16 * ti, H0, d.ldS, 16 * tj, TM));
-    }
+    wgrad_phase<TM, NW, BF, NT, kSlot0, kCnt0>(wacc, wave, wt.n0, 2, Fs, d.ldF, H0, d.ldS,
+                                               part, gp.w0, d.F, d.S);
     // dF = dH0 @ W0^T -> global
     if (gfeat) {
       for (int ct = wave; ct < d.FP / 16; ct += NW) {
         f32x4 acc[RT];
         zero_acc(acc);
-        mma_strip<RT>(H0, d.ldS, kSP, op.g0(), 16 * ct, acc);
+        mma_strip<RT, kSP>(H0, d.ldS, kSP, op.g0(), 16 * ct, acc);
         const int col = 16 * ct + (lane & 15);
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -703,6 +829,39 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d, MlpW w, MlpPack 
           }
       }
     }
+  }
+  // the workgroup's gradient sums -> its partial
+  if constexpr (NT > 0) {
+#pragma unroll
+    for (int si = 0; si < kSlots; ++si) {
+      if (si < kSlot2) {
+        const int t = wave + NW * (si - kSlot3);
+        if (t < wt.n3) store_tile(part, gp.w3, kU, 3, 16 * t, 0, wacc[si]);
+      } else if (si < kSlot1) {
+        const int t = wave + NW * (si - kSlot2);
+        if (t < wt.n2)
+          store_tile(part, gp.w2, kU, kU, 16 * (t / (kU / 16)), 16 * (t % (kU / 16)), wacc[si]);
+      } else if (si < kSlot0) {
+        const int t = wave + NW * (si - kSlot1);
+        if (t < wt.n1)
+          store_tile(part, gp.w1, d.E, kU, 16 * (t / (kU / 16)), 16 * (t % (kU / 16)), wacc[si]);
+      } else {
+        const int t = wave + NW * (si - kSlot0);
+        if (t < wt.n0) store_tile(part, gp.w0, d.F, d.S, 16 * (t / 2), 16 * (t % 2), wacc[si]);
+      }
+    }
+    if (threadIdx.x < kU) {
+      part[gp.b1 + threadIdx.x] = db1;
+      part[gp.b2 + threadIdx.x] = db2;
+    }
+    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] = db3;
+  } else {
+    __syncthreads();
+    if (threadIdx.x < kU) {
+      part[gp.b1 + threadIdx.x] += db1;
+      part[gp.b2 + threadIdx.x] += db2;
+    }
+    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] += db3;
   }
 }
 
@@ -742,7 +901,8 @@ static int num_cus() {
 }
 
 constexpr int kFwdTM = 32, kFwdNW = 4;
-constexpr int kBwdTM = 32, kBwdNW = 8;
+constexpr int kBwdTM = 32;
+constexpr int kBwdNT = kSlots;  // register-resident weight-gradient tiles per wave
 
 int32_t check_mlp(const tfh_mlp* p, const char* who) {
   TFH_REQUIRE(p != nullptr, "%s: null tfh_mlp", who);
@@ -839,15 +999,20 @@ int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   TFH_REQUIRE(lds <= 160 * 1024, "mlp fwd: tile needs %zu bytes of LDS", lds);
   const int64_t blocks = ceil_div(M, kFwdTM);
   ProfScope ps("mlp_fwd_kernel", st);
+  const bool sd = is_std_dims(d, bf);
+#define TFH_LAUNCH_FWD(BFV, STDV)                                                             \
+  do {                                                                                       \
+    allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, BFV, STDV>, lds);                                    \
+    hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW, BFV, STDV>), dim3((unsigned)blocks),       \
+                       dim3(kFwdNW * 64), lds, st, d, to_w(p), pk, feats, dirs, cams, M, rpr,  \
+                       rgb);                                                                 \
+  } while (0)
   if (bf) {
-    allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, true>, lds);
-    hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW, true>), dim3((unsigned)blocks),
-                       dim3(kFwdNW * 64), lds, st, d, to_w(p), pk, feats, dirs, cams, M, rpr, rgb);
+    if (sd) TFH_LAUNCH_FWD(true, true); else TFH_LAUNCH_FWD(true, false);
   } else {
-    allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, false>, lds);
-    hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW, false>), dim3((unsigned)blocks),
-                       dim3(kFwdNW * 64), lds, st, d, to_w(p), pk, feats, dirs, cams, M, rpr, rgb);
+    if (sd) TFH_LAUNCH_FWD(false, true); else TFH_LAUNCH_FWD(false, false);
   }
+#undef TFH_LAUNCH_FWD
   return check_launch("mlp fwd");
 }
 
@@ -882,17 +1047,27 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   TFH_REQUIRE(lds <= 160 * 1024, "mlp bwd: tile needs %zu bytes of LDS", lds);
   {
     ProfScope ps("mlp_bwd", st);
+    // TFH_MLP_WGRAD=partial forces the read-modify-write accumulation of the
+    // weight gradients (the register-resident one is the default when the
+    // shape has a specialised kernel)
+    static const bool force_partial = [] {
+      const char* v = getenv("TFH_MLP_WGRAD");
+      return v && strcmp(v, "partial") == 0;
+    }();
+    const bool sd = is_std_dims(d, bf) && !force_partial;
+#define TFH_LAUNCH_BWD(BFV, NTV, STDV)                                                        \
+  do {                                                                                       \
+    allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV>, lds);                               \
+    hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV>), dim3((unsigned)nblk),    \
+                       dim3(kBwdNW * 64), lds, st, d, to_w(p), pk, gp, feats, dirs, cams,    \
+                       grgb, M, rpr, gfeat, partials, g->embed);                             \
+  } while (0)
     if (bf) {
-      allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, true>, lds);
-      hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, true>), dim3((unsigned)nblk),
-                         dim3(kBwdNW * 64), lds, st, d, to_w(p), pk, gp, feats, dirs, cams, grgb,
-                         M, rpr, gfeat, partials, g->embed);
+      if (sd) TFH_LAUNCH_BWD(true, kBwdNT, true); else TFH_LAUNCH_BWD(true, 0, false);
     } else {
-      allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, false>, lds);
-      hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, false>), dim3((unsigned)nblk),
-                         dim3(kBwdNW * 64), lds, st, d, to_w(p), pk, gp, feats, dirs, cams, grgb,
-                         M, rpr, gfeat, partials, g->embed);
+      if (sd) TFH_LAUNCH_BWD(false, kBwdNT, true); else TFH_LAUNCH_BWD(false, 0, false);
     }
+#undef TFH_LAUNCH_BWD
   }
   TFH_TRY(check_launch("mlp bwd"));
   ProfScope ps("mlp_reduce", st);

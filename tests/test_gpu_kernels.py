@@ -222,3 +222,36 @@ def test_abi_errors(lib):
     vm = lib.Vm(None, None, 10, 4)
     with pytest.raises(lib.TfhError, match="null plane"):
         lib.call("tfh_vm_interpolate", ctypes.byref(vm), None, 5, None, lib.stream())
+
+
+@pytest.mark.parametrize("ncam", [None, 5])
+def test_mlp_bf16_fwd_bwd(lib, ncam):
+    """BF16 operands on MFMA, FP32 accumulation: 2e-2 absolute on RGB; gradients
+    within 5e-2 of the largest float64 gradient of each parameter."""
+    log(f"case: mlp bf16 ncam={ncam}")
+    shape = orc.MlpShape(num_cameras=ncam)
+    p = oracle_params(4, 16, 48, shape, seed=3, bias_std=0.1)
+    _, _, mp = to_device_params(p, shape)
+    mlp = mlp_module(shape, "bf16")
+    rng = np.random.default_rng(12)
+    m = 777
+    f = f32(rng.normal(0, 0.3, size=(m, 144)))
+    d = rng.normal(size=(m, 3))
+    d = f32(d / np.linalg.norm(d, axis=1, keepdims=True))
+    cams = rng.integers(0, ncam or 1, size=m)
+    for leaf in mlp.flat_leaves(mp):
+        leaf.requires_grad_(True)
+    tf = dev(f).requires_grad_(True)
+    rgb = mlp.apply(mp, tf, dev(d), dev(cams, torch.int32))
+    want, cache = orc.feature_mlp_forward(p, f, d, cams, shape)
+    err = report("rgb (bf16)", rgb.detach().cpu().numpy(), want)
+    assert err.max() < 2e-2
+    g = f32(rng.normal(size=(m, 3)))
+    (rgb * dev(g)).sum().backward()
+    grads, gf = orc.feature_mlp_backward(p, cache, g, shape)
+    ef = report("grad features (bf16)", tf.grad.cpu().numpy(), gf)
+    assert ef.max() < 5e-2 * np.abs(gf).max()
+    names = ["w0", "w1", "b1", "w2", "b2", "w3", "b3"] + (["embed"] if ncam else [])
+    for name, leaf in zip(names, mlp.flat_leaves(mp)):
+        e = report(f"grad {name} (bf16)", leaf.grad.cpu().numpy(), grads[name])
+        assert e.max() < 5e-2 * np.abs(grads[name]).max() + 1e-6, name

@@ -189,3 +189,19 @@ def test_train_step_matches_oracle():
         frac_bad = np.mean(e > 1e-3 * lr)
         log(f"  fraction of {key} updates off by >1e-3 lr: {frac_bad:.2e}")
         assert frac_bad < 1e-3
+
+
+def test_render_rgb_bf16_mlp():
+    """BF16 MLP in the full render (config-3 style top-k): 2e-2 absolute RGB."""
+    log("case: render rgb bf16 N=128 k=16")
+    from tensorf_amd import render
+    shape, p, o, d, cams, box = _setup(r=256)
+    lp, rays, mlp = _gpu_objects(shape, p, o, d, cams, False, precision="bf16")
+    cfg = render.RenderConfig(0.05, 200.0, render.RenderMode.RGB, 128, 16)
+    with torch.no_grad():
+        out = render.render_rays(mlp, lp, dev(box), rays, 5, cfg).cpu().numpy()
+    jit = _oracle_jitter(5, len(o), 128, False)
+    want, cache = orc.render_rays(p, box, o, d, cams, orc.RenderConfig(0.05, 200.0, orc.RGB, 128, 16),
+                                  shape, jit)
+    err = report("rgb (bf16 mlp)", out, want)
+    assert err.max() < 2e-2
