@@ -238,6 +238,19 @@ int32_t tfh_train_step(const tfh_vm* density, const tfh_vm* appearance, const tf
                        float* mse_out, void* workspace, size_t workspace_bytes,
                        tfh_stream_t stream);
 
+/* ---- synthetic scenes ----------------------------------------------------- */
+/* Ground truth of a procedural scene (replaces the image datasets of data.py):
+ * density = sum of isotropic Gaussian blobs, blobs[b] = {cx, cy, cz, sigma,
+ * peak, r, g, b} (float64), colour = density-weighted albedo, white
+ * background, emission-absorption midpoint quadrature in FP64 with
+ * samples_per_ray samples over [t_near, t_far] (uniform in t, or with
+ * contracted_far != 0 uniform in t up to t=1 and uniform in 1/t beyond).
+ * rgb (R,3); opacity (R) may be NULL. No counterpart in the reference. */
+int32_t tfh_blob_scene_render(const double* blobs, int32_t num_blobs, const float* origins,
+                              const float* directions, int64_t num_rays, double t_near,
+                              double t_far, int32_t samples_per_ray, int32_t contracted_far,
+                              float* rgb, float* opacity, tfh_stream_t stream);
+
 /* ---- stage timing --------------------------------------------------------- */
 /* Per-stage timing with HIP events recorded on the launching stream around
  * every kernel the library launches. tfh_profile_enable(1) clears and starts

@@ -366,18 +366,33 @@ def init_mlp_params(shape: MlpShape, rng: np.random.Generator):
     return p
 
 
+def bf16_round(x):
+    """Round to the nearest bfloat16 (ties to even), returned as float64.
+    Emulates the BF16 MFMA operands of the GPU MLP (FP32 accumulation)."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _mm(a, b, rnd):
+    return (rnd(a) @ rnd(b)) if rnd is not None else a @ b
+
+
 def feature_mlp_forward(params, features, viewdirs, camera_indices, shape: MlpShape,
-                        dtype=np.float64):
-    """FeatureMlp.__call__, networks.py:46-121. Returns (rgb (M,3), cache)."""
+                        dtype=np.float64, operand_round=None):
+    """FeatureMlp.__call__, networks.py:46-121. Returns (rgb (M,3), cache).
+    operand_round (e.g. bf16_round) is applied to both operands of every
+    matrix product, to model reduced-precision MFMA inputs."""
+    rnd = operand_round
     w0 = _c(params["w0"], dtype)
     f = _c(features, dtype)
     d = _c(viewdirs, dtype)
-    h0 = f @ w0                                                      # networks.py:57-61
+    h0 = _mm(f, w0, rnd)                                             # networks.py:57-61
     x = np.concatenate([h0, d, fourier_encode(h0, shape.feature_n_freqs, dtype),
                         fourier_encode(d, shape.viewdir_n_freqs, dtype)], axis=-1)  # networks.py:68-76
-    z1 = x @ _c(params["w1"], dtype) + _c(params["b1"], dtype)       # networks.py:86-89
+    z1 = _mm(x, _c(params["w1"], dtype), rnd) + _c(params["b1"], dtype)   # networks.py:86-89
     h1 = np.maximum(z1, 0)                                           # networks.py:90
-    z2 = h1 @ _c(params["w2"], dtype) + _c(params["b2"], dtype)      # networks.py:94-97
+    z2 = _mm(h1, _c(params["w2"], dtype), rnd) + _c(params["b2"], dtype)  # networks.py:94-97
     h2 = np.maximum(z2, 0)                                           # networks.py:98
     h2f = h2
     emb = None
@@ -386,10 +401,10 @@ def feature_mlp_forward(params, features, viewdirs, camera_indices, shape: MlpSh
         half = shape.units // 2
         h2f = h2.copy()
         h2f[:, half:] = emb[:, :half] * h2[:, half:] + emb[:, half:]
-    z3 = h2f @ _c(params["w3"], dtype) + _c(params["b3"], dtype)     # networks.py:114-117
+    z3 = _mm(h2f, _c(params["w3"], dtype), rnd) + _c(params["b3"], dtype)  # networks.py:114-117
     rgb = sigmoid(z3)                                                # networks.py:120
     cache = dict(f=f, d=d, h0=h0, x=x, z1=z1, h1=h1, z2=z2, h2=h2, h2f=h2f, emb=emb,
-                 rgb=rgb, cams=np.asarray(camera_indices, np.int64))
+                 rgb=rgb, cams=np.asarray(camera_indices, np.int64), rnd=rnd)
     return rgb.astype(dtype), cache
 
 
@@ -397,10 +412,11 @@ def feature_mlp_backward(params, cache, grad_rgb, shape: MlpShape, dtype=np.floa
     """Hand-derived VJP of FeatureMlp w.r.t. its parameters and input features.
     Returns (param_grads dict, grad_features (M, feat_dim))."""
     g = _c(grad_rgb, dtype)
+    rnd = cache.get("rnd")
     rgb = cache["rgb"]
     dz3 = g * rgb * (1.0 - rgb)
-    grads = {"w3": cache["h2f"].T @ dz3, "b3": dz3.sum(0)}
-    dh2f = dz3 @ _c(params["w3"], dtype).T
+    grads = {"w3": _mm(cache["h2f"].T, dz3, rnd), "b3": dz3.sum(0)}
+    dh2f = _mm(dz3, _c(params["w3"], dtype).T, rnd)
     dh2 = dh2f
     if shape.num_cameras is not None:
         half = shape.units // 2
@@ -413,13 +429,13 @@ def feature_mlp_backward(params, cache, grad_rgb, shape: MlpShape, dtype=np.floa
         np.add.at(gemb, cache["cams"], demb_rows)
         grads["embed"] = gemb
     dz2 = dh2 * (cache["z2"] > 0)
-    grads["w2"] = cache["h1"].T @ dz2
+    grads["w2"] = _mm(cache["h1"].T, dz2, rnd)
     grads["b2"] = dz2.sum(0)
-    dh1 = dz2 @ _c(params["w2"], dtype).T
+    dh1 = _mm(dz2, _c(params["w2"], dtype).T, rnd)
     dz1 = dh1 * (cache["z1"] > 0)
-    grads["w1"] = cache["x"].T @ dz1
+    grads["w1"] = _mm(cache["x"].T, dz1, rnd)
     grads["b1"] = dz1.sum(0)
-    dx = dz1 @ _c(params["w1"], dtype).T
+    dx = _mm(dz1, _c(params["w1"], dtype).T, rnd)
     sq = shape.squash_dim
     nf = shape.feature_n_freqs
     h0 = cache["h0"]
@@ -429,8 +445,8 @@ def feature_mlp_backward(params, cache, grad_rgb, shape: MlpShape, dtype=np.floa
         s = 2.0 ** q
         dh0 += fe[:, :, q] * s * np.cos(s * h0)
         dh0 += fe[:, :, nf + q] * s * np.cos(s * h0 + 0.5 * np.pi)
-    grads["w0"] = cache["f"].T @ dh0
-    gfeat = dh0 @ _c(params["w0"], dtype).T
+    grads["w0"] = _mm(cache["f"].T, dh0, rnd)
+    gfeat = _mm(dh0, _c(params["w0"], dtype).T, rnd)
     return {k: v.astype(dtype) for k, v in grads.items()}, gfeat.astype(dtype)
 
 

@@ -1,0 +1,152 @@
+// MFMA tile helpers shared by the MLP kernels (gfx950, wave64).
+//
+// D layout of a 16x16 tile: lane l, element r -> row 4*(l/16)+r, col l%16.
+// FP32 (K=4):  A[l%16][k0 + l/16],        B[k0 + l/16][l%16]
+// BF16 (K=32): A[l%16][k0 + 8(l/16) + e], B[k0 + 8(l/16) + e][l%16], e < 8
+#pragma once
+
+#include "tfh_common.h"
+
+namespace tfh {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+
+template <int NR>
+__device__ __forceinline__ void zero_acc(f32x4 (&acc)[NR]) {
+#pragma unroll
+  for (int i = 0; i < NR; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+}
+
+__device__ __forceinline__ bf16x8 to_bf16x8(const float4& lo, const float4& hi) {
+  bf16x8 v;
+  v[0] = (__bf16)lo.x; v[1] = (__bf16)lo.y; v[2] = (__bf16)lo.z; v[3] = (__bf16)lo.w;
+  v[4] = (__bf16)hi.x; v[5] = (__bf16)hi.y; v[6] = (__bf16)hi.z; v[7] = (__bf16)hi.w;
+  return v;
+}
+
+// Raw buffer resource over `bytes` bytes at `base`: loads take a 32-bit VGPR
+// byte offset (no per-load 64-bit scalar address) and read 0 past the end.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+constexpr int kOOB = 0x7ffffff0;  // a byte offset past every buffer: loads return 0
+
+// Row-major FP32 global matrix as a B operand with zero padding outside (rows, cols).
+struct GlobalB {
+  __amdgpu_buffer_rsrc_t r;
+  int rows, cols;
+  __device__ __forceinline__ GlobalB(const float* w, int rows_, int cols_)
+      : r(buffer_rsrc(w, rows_ * cols_ * 4)), rows(rows_), cols(cols_) {}
+  __device__ __forceinline__ float operator()(int k, int j) const {
+    const int off = (k < rows && j < cols) ? (k * cols + j) * 4 : kOOB;
+    // the builtin returns the raw 32 bits as an integer: reinterpret, do not convert
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  }
+};
+
+// BF16 table T[j][k] (stride ld) as a B operand: B(k, j) = T[j][k].
+struct PackedB {
+  __amdgpu_buffer_rsrc_t r;
+  int ld;
+  __device__ __forceinline__ PackedB(const __bf16* t, int ld_, int rows)
+      : r(buffer_rsrc(t, rows * ld_ * 2)), ld(ld_) {}
+  __device__ __forceinline__ bf16x8 load8(int k, int j) const {
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    const i32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (j * ld + k) * 2, 0, 0);
+    return __builtin_bit_cast(bf16x8, v);
+  }
+};
+
+// acc[rt] += A(rows 16*rt.., K) * B(K, cols col0..col0+15), rt < NR; A from
+// LDS (row-major, lda), K <= KMAX. The whole B column strip is loaded into
+// registers first (KMAX/4 floats or KMAX/32 bf16x8 per lane) so its L2
+// latency overlaps; A is read from LDS per k-step.
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, GlobalB b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  const int lane = lane_id();
+  const int r = lane & 15, kq = lane >> 4;
+  constexpr int NS = KMAX / 4;
+  for (int kc = 0; kc < K; kc += KMAX) {  // one pass for K <= KMAX
+    float bb[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) bb[i] = (kc + 4 * i < K) ? b(kc + 4 * i + kq, col0 + r) : 0.0f;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      if (kc + 4 * i < K) {
+#pragma unroll
+        for (int rt = 0; rt < NR; ++rt) {
+          const float a = A[(16 * rt + r) * lda + kc + 4 * i + kq];
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb[i], acc[rt], 0, 0, 0);
+        }
+      }
+    }
+  }
+}
+
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  const int lane = lane_id();
+  const int r = lane & 15, kq = lane >> 4;
+  constexpr int NS = (KMAX + 31) / 32;
+  for (int kc = 0; kc < K; kc += 32 * NS) {  // one pass for K <= KMAX
+    bf16x8 bb[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+      if (kc + 32 * i < K) bb[i] = b.load8(kc + 32 * i + 8 * kq, col0 + r);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      if (kc + 32 * i < K) {
+#pragma unroll
+        for (int rt = 0; rt < NR; ++rt) {
+          const float* ap = A + (16 * rt + r) * lda + kc + 32 * i + 8 * kq;
+          const bf16x8 av = to_bf16x8(*reinterpret_cast<const float4*>(ap),
+                                      *reinterpret_cast<const float4*>(ap + 4));
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bb[i], acc[rt], 0, 0, 0);
+        }
+      }
+    }
+  }
+}
+
+// Weight-gradient tile: acc(i, j) = sum_k X[k][row0 + i] * Y[k][col0 + j],
+// k < K (the rows of the tile).
+template <bool BF, int K>
+__device__ __forceinline__ void mma_wgrad_acc(const float* __restrict__ X, int ldx, int row0,
+                                              const float* __restrict__ Y, int ldy, int col0,
+                                              f32x4& acc) {
+  const int lane = lane_id();
+  const int r = lane & 15, kq = lane >> 4;
+  if constexpr (BF) {
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 32) {
+      bf16x8 av, bv;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        av[e] = (__bf16)X[(k0 + 8 * kq + e) * ldx + row0 + r];
+        bv[e] = (__bf16)Y[(k0 + 8 * kq + e) * ldy + col0 + r];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const float a = X[(k0 + kq) * ldx + row0 + r];
+      const float b = Y[(k0 + kq) * ldy + col0 + r];
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+  }
+}
+
+template <bool BF, int K>
+__device__ __forceinline__ f32x4 mma_wgrad(const float* __restrict__ X, int ldx, int row0,
+                                           const float* __restrict__ Y, int ldy, int col0) {
+  f32x4 acc{0.f, 0.f, 0.f, 0.f};
+  mma_wgrad_acc<BF, K>(X, ldx, row0, Y, ldy, col0, acc);
+  return acc;
+}
+
+}  // namespace tfh

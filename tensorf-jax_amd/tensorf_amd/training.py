@@ -246,3 +246,44 @@ class TrainState:
 
     def psnr(self) -> float:
         return utils.psnr_from_mse(self._mse.item())
+
+
+def upsampled_grid_dim(config: TensorfConfig, upsamp_index: int) -> int:
+    """training.py:360-368: linear interpolation from grid_dim_init to grid_dim_final."""
+    return int(config.grid_dim_init + (config.grid_dim_final - config.grid_dim_init)
+               * ((upsamp_index + 1) / len(config.upsamp_iters)))
+
+
+def run_training_loop(config: TensorfConfig, dataset, n_iters: Optional[int] = None,
+                      log_every: int = 100, prng_key: int = 94709, log_fn=None) -> TrainState:
+    """training.py:279-368 without the experiment harness: cycled shuffled
+    minibatches, one training step per iteration, grid upsampling (with the
+    Adam moments) at config.upsamp_iters. Logs are device scalars read every
+    `log_every` steps (the reference syncs on the step counter every step)."""
+    state = TrainState.initialize(config, config.grid_dim_init, prng_key,
+                                  num_cameras=dataset.n_views)
+    batches = dataset.minibatches(config.minibatch_size, seed=0)
+    total = n_iters if n_iters is not None else config.n_iters
+    for it in range(total):
+        state, logs = state.training_step(next(batches))
+        step = state.step
+        if log_fn is not None and (step % log_every == 0 or step == total):
+            log_fn(step, float(logs["mse"].item()), logs)
+        if step in config.upsamp_iters:
+            state = state.resize_grid(upsampled_grid_dim(config, config.upsamp_iters.index(step)))
+    return state
+
+
+def evaluate_psnr(state: TrainState, dataset, views, mode=None, chunk: int = 65536) -> float:
+    """Mean PSNR of rendered views against the dataset's ground truth
+    (render_rays_batched over each view, render.py:49-102)."""
+    cfg = state.render_config()
+    vals = []
+    for v in views:
+        rr = dataset.view(v)
+        out = render.render_rays_batched(state.appearance_mlp, state.learnable_params, state.aabb,
+                                         rr.rays_wrt_world, 0, cfg, batch_size=chunk)
+        pred = torch.tensor(out, device=rr.colors.device)
+        mse = float(((pred - rr.colors) ** 2).mean())
+        vals.append(utils.psnr_from_mse(mse))
+    return sum(vals) / len(vals)

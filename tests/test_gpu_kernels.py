@@ -226,8 +226,10 @@ def test_abi_errors(lib):
 
 @pytest.mark.parametrize("ncam", [None, 5])
 def test_mlp_bf16_fwd_bwd(lib, ncam):
-    """BF16 operands on MFMA, FP32 accumulation: 2e-2 absolute on RGB; gradients
-    within 5e-2 of the largest float64 gradient of each parameter."""
+    """BF16 operands on MFMA, FP32 accumulation. Checked against the oracle with
+    the same BF16 operand rounding (tight: 2e-3 of the largest value, the FP32
+    accumulation order and the rounding of near-ties differ), and reported
+    against float64 (bound: 2e-2 absolute on RGB)."""
     log(f"case: mlp bf16 ncam={ncam}")
     shape = orc.MlpShape(num_cameras=ncam)
     p = oracle_params(4, 16, 48, shape, seed=3, bias_std=0.1)
@@ -243,15 +245,17 @@ def test_mlp_bf16_fwd_bwd(lib, ncam):
         leaf.requires_grad_(True)
     tf = dev(f).requires_grad_(True)
     rgb = mlp.apply(mp, tf, dev(d), dev(cams, torch.int32))
-    want, cache = orc.feature_mlp_forward(p, f, d, cams, shape)
-    err = report("rgb (bf16)", rgb.detach().cpu().numpy(), want)
-    assert err.max() < 2e-2
+    want64, _ = orc.feature_mlp_forward(p, f, d, cams, shape)
+    want, cache = orc.feature_mlp_forward(p, f, d, cams, shape, operand_round=orc.bf16_round)
+    got = rgb.detach().cpu().numpy()
+    assert report("rgb (bf16) vs float64", got, want64).max() < 2e-2
+    assert report("rgb (bf16) vs bf16-operand oracle", got, want).max() < 2e-3
     g = f32(rng.normal(size=(m, 3)))
     (rgb * dev(g)).sum().backward()
     grads, gf = orc.feature_mlp_backward(p, cache, g, shape)
     ef = report("grad features (bf16)", tf.grad.cpu().numpy(), gf)
-    assert ef.max() < 5e-2 * np.abs(gf).max()
+    assert ef.max() < 2e-2 * np.abs(gf).max()
     names = ["w0", "w1", "b1", "w2", "b2", "w3", "b3"] + (["embed"] if ncam else [])
     for name, leaf in zip(names, mlp.flat_leaves(mp)):
         e = report(f"grad {name} (bf16)", leaf.grad.cpu().numpy(), grads[name])
-        assert e.max() < 5e-2 * np.abs(grads[name]).max() + 1e-6, name
+        assert e.max() < 2e-2 * np.abs(grads[name]).max() + 1e-6, name

@@ -241,3 +241,12 @@ def test_train_step_reduces_loss():
                                               jit, n_iters=100)
         losses.append(logs["mse"])
     assert losses[-1] < 0.9 * losses[0]
+
+
+def test_bf16_round_matches_torch():
+    """The oracle's BF16 operand rounding equals torch's float32 -> bfloat16 cast
+    (round to nearest, ties to even), including exact ties and subnormals."""
+    x = np.concatenate([RNG.normal(size=1000) * 10.0 ** RNG.integers(-30, 30, 1000),
+                        [1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -1.0 - 2.0 ** -8, 1e-40, 0.0]])
+    want = torch.tensor(x, dtype=torch.float32).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(orc.bf16_round(x), want)
