@@ -34,26 +34,36 @@ PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0}
 
 
-def stage_model(stage, R, N, k, cd, ca, precision, n_params):
-    """Algorithmic work of one launch of a stage: (bound, amount, unit).
+def stage_model(stage, R, N, k, cd, ca, precision, n_params, prop=None):
+    """Algorithmic work of one launch of a stage: (bound, amount, unit, peak).
     Bytes: VM taps 72*C per sample (3 modes x 6 taps x C x 4 B) plus the
-    streamed per-sample I/O; FLOPs: 79,712 per MLP row forward (nf=nv=2)."""
+    streamed per-sample I/O; FLOPs: 79,712 per feature-MLP row forward
+    (nf=nv=2) and 11,392 per proposal-MLP row (2 (24*64 + 64*64 + 64)).
+    prop: (samples per ray over the proposal rounds, proposal channels)."""
     M = R * k
     mlp_row = 2 * (144 * 27 + 150 * 128 + 128 * 128 + 128 * 3)
+    mfma_peak = PEAK_TFLOPS[precision]
     if stage == "density_pass":
-        return "hbm", R * N * (72 * cd + 28), "GB/s"
+        return "hbm", R * N * (72 * cd + 28), "GB/s", PEAK_HBM_GBS
     if stage == "appearance_gather":
-        return "hbm", M * (72 * ca + 12 + 4 * 3 * ca), "GB/s"
+        return "hbm", M * (72 * ca + 12 + 4 * 3 * ca), "GB/s", PEAK_HBM_GBS
     if stage == "mlp_fwd_kernel":
-        return "mfma", M * mlp_row, "TFLOP/s"
+        return "mfma", M * mlp_row, "TFLOP/s", mfma_peak
     if stage == "mlp_bwd":
-        return "mfma", M * 3 * mlp_row, "TFLOP/s"
+        return "mfma", M * 3 * mlp_row, "TFLOP/s", mfma_peak
     if stage == "composite_bwd_density_scatter":
-        return "hbm", R * N * (2 * 72 * cd + 36), "GB/s"
+        return "hbm", R * N * (2 * 72 * cd + 36), "GB/s", PEAK_HBM_GBS
     if stage == "appearance_scatter":
-        return "hbm", M * (2 * 72 * ca + 12 + 4 * 3 * ca), "GB/s"
+        return "hbm", M * (2 * 72 * ca + 12 + 4 * 3 * ca), "GB/s", PEAK_HBM_GBS
     if stage == "adam":
-        return "hbm", n_params * 32, "GB/s"
+        return "hbm", n_params * 32, "GB/s", PEAK_HBM_GBS
+    if prop is not None:
+        ns, cp = prop
+        prow = 2 * (3 * cp * 64 + 64 * 64 + 64)
+        if stage == "prop_mlp_fwd":
+            return "mfma", R * ns * prow, "TFLOP/s", PEAK_TFLOPS["fp32"]
+        if stage == "prop_mlp_bwd":
+            return "mfma", R * ns * 3 * prow, "TFLOP/s", PEAK_TFLOPS["fp32"]
     return None
 
 
@@ -102,6 +112,75 @@ def cpu_baseline(R_sample, N, k, G, cd, ca):
                       f"{dt:.1f} s"}
 
 
+def bench_proposal(args, cd, ca, G):
+    """Proposal-sampled training and rendering: R rays, proposal rounds 128 ->
+    64 on a 64^3 x 8-channel proposal field, 64 field samples per ray on the
+    proposal's intervals, all 64 shaded by the BF16 feature MLP, Adam on both
+    parameter sets. Returns the numbers merged into the JSON line."""
+    import torch
+    from tensorf_amd import _lib, cameras, proposal, render, training
+    R = args.prop_rays
+    pc = proposal.ProposalConfig(samples_round0=128, samples_round1=64, samples_out=64)
+    cfg = training.TensorfConfig(
+        minibatch_size=R, n_iters=30000, initial_aabb_min=(-1.5,) * 3,
+        initial_aabb_max=(1.5,) * 3, appearance_feat_dim=ca, density_feat_dim=cd,
+        feature_n_freqs=2, viewdir_n_freqs=2, grid_dim_init=G, grid_dim_final=G, upsamp_iters=(),
+        appearance_samples_per_ray=64, mlp_precision="bf16", proposal=pc)
+    progress(f"proposal: init state G={G} R={R} 128->64->64 bf16")
+    state = training.TrainState.initialize(cfg, G, prng_key=0, num_cameras=1)
+    batches = make_batches(R, 4, seed=5, device="cuda")
+    cams = torch.zeros(R, dtype=torch.int32, device="cuda")
+    mbs = [cameras.RenderedRays(c, cameras.Rays3D(o, d, cams)) for o, d, c in batches]
+    for i in range(args.warmup):
+        state.training_step(mbs[i % 4])
+        torch.cuda.synchronize()
+    progress("proposal: warmup done")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        state.training_step(mbs[i % 4])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    train = R * args.steps / dt
+    progress(f"proposal: {1e3 * dt / args.steps:.3f} ms/step, {train:.1f} rays/s")
+    lp = state.learnable_params
+    rcfg = state.render_config()
+    rays = [m.rays_wrt_world for m in mbs]
+    ws = {}
+
+    def render_once(i):
+        env = render._RenderEnv(state.appearance_mlp, lp, state.aabb, rays[i % 4], 7, rcfg,
+                                ws.get("w"))
+        ws["w"] = env.ws
+        env.forward()
+
+    for i in range(2):
+        render_once(i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    n_render = max(3, args.steps // 2)
+    for i in range(n_render):
+        render_once(i)
+    torch.cuda.synchronize()
+    rend = R * n_render / (time.perf_counter() - t1)
+    progress(f"proposal render: {rend:.1f} rays/s")
+    stages = {}
+    if not args.no_stages:
+        _lib.profile_enable(True)
+        for i in range(3):
+            state.training_step(mbs[i % 4])
+        torch.cuda.synchronize()
+        rec = _lib.profile_read()
+        _lib.profile_enable(False)
+        stages = {name: round(ms / max(cnt, 1), 4) for name, (ms, cnt) in rec.items()}
+        progress("proposal stages: " + ", ".join(f"{k}={v:.3f}ms" for k, v in stages.items()))
+    return {"workload": f"proposal-sampled training step: R={R} rays, proposal 64^3 x 8 "
+                        f"channels + 2x64 FP32 MLP, rounds 128 -> 64, 64 field samples on the "
+                        f"proposal intervals all shaded by the BF16 MLP, VM {G}^3, Adam on both",
+            "train_rays_per_s": round(train, 1), "ms_per_step": round(1e3 * dt / args.steps, 3),
+            "render_rays_per_s": round(rend, 1), "stages_ms": stages,
+            "final_mse": float(state._mse.item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +194,8 @@ def main():
     ap.add_argument("--cpu-sample-rays", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stages", action="store_true")
+    ap.add_argument("--prop-rays", type=int, default=16384)
+    ap.add_argument("--no-proposal", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -199,13 +280,8 @@ def main():
             if model is not None and (best is None or avg > best[1]):
                 best = (name, avg, model)
         if best is not None:
-            name, avg, (bound, amount, unit) = best
-            if unit == "GB/s":
-                achieved = amount / (avg * 1e-3) / 1e9
-                peak = PEAK_HBM_GBS
-            else:
-                achieved = amount / (avg * 1e-3) / 1e12
-                peak = PEAK_TFLOPS[args.precision]
+            name, avg, (bound, amount, unit, peak) = best
+            achieved = amount / (avg * 1e-3) / (1e9 if unit == "GB/s" else 1e12)
             traffic = None
             tpath = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
             if os.path.exists(tpath):
@@ -217,6 +293,14 @@ def main():
                         "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
                         "traffic": traffic, "avg_ms": round(avg, 4),
                         "algorithmic_per_launch": amount}
+
+    prop = None
+    if not args.no_proposal:
+        del state, mbs, batches, rays, lp, ws
+        torch.cuda.empty_cache()
+        prop = bench_proposal(args, cd, ca, G)
+        prop["train_speedup_vs_this_config"] = round(prop["train_rays_per_s"] / value, 2)
+        prop["render_speedup_vs_this_config"] = round(prop["render_rays_per_s"] / render_value, 2)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -246,6 +330,7 @@ def main():
         "stages_ms": stages,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "proposal": prop,
     }
     print(json.dumps(line), flush=True)
 

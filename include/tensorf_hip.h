@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define TFH_ABI_VERSION 1
+#define TFH_ABI_VERSION 2
 
 typedef void* tfh_stream_t;
 
@@ -53,6 +53,18 @@ enum tfh_render_mode { TFH_MODE_RGB = 0, TFH_MODE_DIST_MEDIAN = 1, TFH_MODE_DIST
 /* Arithmetic of the appearance MLP. FP32: FP32 MFMA. BF16: BF16 operands on
  * MFMA with FP32 accumulation. */
 enum tfh_precision { TFH_PREC_FP32 = 0, TFH_PREC_BF16 = 1 };
+
+/* Where the N density samples of a ray lie.
+ * SCHEDULE: the reference's samplers -- stratified in the box segment
+ *   (render.py:350-390) or, with scene_contraction, the contracted schedule
+ *   base_ts / base_steps (render.py:122-173).
+ * EDGES: per-ray interval edges e (R, N+1), non-decreasing along each ray,
+ *   passed in the base_ts argument (base_steps unused); sample i lies at
+ *   t_i = e_i + u_i (e_{i+1} - e_i) with u = jitter (R, N) (0.5 when jitter is
+ *   NULL) and has step size e_{i+1} - e_i. This is how a proposal sampler
+ *   (tfh_proposal_sample) hands its intervals to the field. No counterpart in
+ *   the reference. */
+enum tfh_sampling { TFH_SAMPLING_SCHEDULE = 0, TFH_SAMPLING_EDGES = 1 };
 
 /* One VM-decomposed field: TensorVM (tensor_vm.py:14-19), channel-last. */
 typedef struct {
@@ -100,7 +112,7 @@ typedef struct {
   int32_t appearance_samples_per_ray; /* k, 1..N */
   int32_t scene_contraction;          /* 0: bounded box, 1: L-inf contraction */
   int32_t mlp_precision;              /* tfh_precision */
-  int32_t reserved;
+  int32_t sampling;                   /* tfh_sampling */
 } tfh_render_config;
 
 /* ---- library ------------------------------------------------------------ */
@@ -153,6 +165,18 @@ int32_t tfh_sample_contracted(const float* origins, const float* directions, int
 int32_t tfh_segment_probabilities(const float* sigmas, const float* step_sizes, int64_t num_rays,
                                   int64_t samples_per_ray, float* p_exits, float* p_terminates,
                                   tfh_stream_t stream);
+/* VJP of compute_segment_probabilities w.r.t. the densities (the reference
+ * differentiates it inside jax.value_and_grad, training.py:121-134):
+ * with a_i = sigma_i delta_i,
+ *   dL/da_i = gw_i E_i - sum_{j>i} gw_j w_j - sum_{j>=i} ge_j E_j,
+ *   grad_sigmas_i = dL/da_i delta_i.
+ * p_exits / p_terminates: the forward's outputs; either upstream gradient
+ * (R, N) may be NULL, not both. sigmas is not read (may be NULL). */
+int32_t tfh_segment_probabilities_bwd(const float* sigmas, const float* step_sizes,
+                                      int64_t num_rays, int64_t samples_per_ray,
+                                      const float* p_exits, const float* p_terminates,
+                                      const float* grad_p_exits, const float* grad_p_terminates,
+                                      float* grad_sigmas, tfh_stream_t stream);
 /* Appearance sample selection (render.py:459-470): per ray the k indices of
  * the largest p_terminates (log p + gumbel when gumbel != NULL; gumbel is
  * (N,) shared by all rays), ties to the lower index, written ascending. */
@@ -237,6 +261,143 @@ int32_t tfh_train_step(const tfh_vm* density, const tfh_vm* appearance, const tf
                        float lr_tensor, float lr_mlp, float decay, int64_t adam_count,
                        float* mse_out, void* workspace, size_t workspace_bytes,
                        tfh_stream_t stream);
+
+/* ---- proposal sampler ------------------------------------------------------
+ * A learned proposal density field that places the field's samples. The
+ * reference stops at a comment (render.py:126: "An occupancy grid or proposal
+ * network would really help us here!"); nothing below has a reference
+ * counterpart. The field is a VM decomposition (typically 64^3, C = 8 per
+ * mode) whose 3C features feed a small MLP:
+ *   z = w3 . relu(W2^T relu(W1^T f + b1) + b2) + b3,  sigma = softplus(z)
+ * (kernels (in, out) row-major like the FeatureMlp's, hidden width 64).
+ * Sampling runs rounds: round 0 takes samples_round0 strata of the
+ * reference's sampler (the box segment split evenly, or the contracted
+ * schedule); each round evaluates the proposal density at its samples,
+ * composites their weights w_j over its intervals (the reference's
+ * compute_segment_probabilities, render.py:300-347) and places the next
+ * round's interval edges at the quantiles m/n' of the piecewise-linear CDF
+ * of (w_j + pad / n_j) (the edges of the round's span are kept). The last
+ * round's edges (R, samples_out + 1) are the field's intervals
+ * (TFH_SAMPLING_EDGES). Training (mip-NeRF 360's interval loss) takes the
+ * field's weights w_i on its intervals as constants and, for every round,
+ * bound_i = sum of the round's w_j over the intervals overlapping interval i,
+ *   L = weight / R * sum_rays sum_i max(0, w_i - bound_i)^2 / (w_i + 1e-7),
+ * differentiated w.r.t. the proposal parameters only. */
+typedef struct {
+  const float* w1; /* (feat_dim, 64) */
+  const float* b1; /* (64) */
+  const float* w2; /* (64, 64) */
+  const float* b2; /* (64) */
+  const float* w3; /* (64, 1) */
+  const float* b3; /* (1) */
+  int64_t feat_dim; /* 3 * proposal channels, 1..32 */
+} tfh_prop_mlp;
+
+typedef struct {
+  float *w1, *b1, *w2, *b2, *w3, *b3; /* accumulated into (+=) */
+} tfh_prop_mlp_grad;
+
+typedef struct {
+  int32_t samples_round0;   /* strata of round 0, 2..1024 */
+  int32_t samples_round1;   /* samples of round 1 (0 = one round only), <= 1024 */
+  int32_t samples_out;      /* intervals handed to the field, 1..1024 */
+  int32_t scene_contraction; /* as tfh_render_config */
+  float histogram_pad;      /* pad mass added to each round's weights before resampling */
+  float loss_weight;        /* weight of the interval loss */
+} tfh_proposal_config;
+
+/* The proposal MLP alone: features (M, feat_dim) -> z (M) (may be NULL) and
+ * sigma = softplus(z) (M). */
+int32_t tfh_prop_mlp_fwd(const tfh_prop_mlp* mlp, const float* features, int64_t num_rows,
+                         float* z, float* sigma, tfh_stream_t stream);
+size_t tfh_prop_mlp_bwd_workspace_size(const tfh_prop_mlp* mlp, int64_t num_rows);
+/* VJP of tfh_prop_mlp_fwd w.r.t. z: grad_z (M) -> grad_features (M,
+ * feat_dim) (may be NULL) and parameter gradients accumulated into grad. */
+int32_t tfh_prop_mlp_bwd(const tfh_prop_mlp* mlp, const float* features, int64_t num_rows,
+                         const float* grad_z, float* grad_features,
+                         const tfh_prop_mlp_grad* grad, void* workspace, size_t workspace_bytes,
+                         tfh_stream_t stream);
+
+/* Inverse-CDF resampling of one round: for each ray, edges (n+1) and
+ * weights (n) -> edges_out (num_out+1) at the quantiles m/num_out of the
+ * piecewise-linear CDF of (w_j + pad/n) over the edges; edges_out[0] and
+ * edges_out[num_out] repeat the ray's first and last edge. */
+int32_t tfh_proposal_resample(const float* edges, const float* weights, int64_t num_rays,
+                              int64_t num_intervals, int64_t num_out, float pad, float* edges_out,
+                              tfh_stream_t stream);
+/* Interval loss per ray (see above) of field intervals edges (R, n+1) with
+ * weights (R, n) against proposal intervals prop_edges (R, n_p+1) with
+ * prop_weights (R, n_p), times scale: loss (R) and, when grad_prop_weights
+ * != NULL, its gradient w.r.t. prop_weights (R, n_p). */
+int32_t tfh_interval_loss(const float* edges, const float* weights, int64_t num_intervals,
+                          const float* prop_edges, const float* prop_weights,
+                          int64_t num_prop_intervals, int64_t num_rays, float scale, float* loss,
+                          float* grad_prop_weights, tfh_stream_t stream);
+
+/* Workspace of tfh_proposal_sample / tfh_proposal_loss_bwd for R rays. */
+size_t tfh_proposal_workspace_size(const tfh_proposal_config* pc, const tfh_vm* field,
+                                   int64_t num_rays);
+/* Proposal rounds for R rays. jitter: (R, samples_round0 + samples_round1)
+ * fractions in [0,1) placing each round's samples in their intervals (NULL:
+ * midpoints). base_ts / base_steps: the contracted schedule of
+ * samples_round0 entries when scene_contraction (render.py:127-155), else
+ * NULL. edges_out: (R, samples_out + 1). The workspace keeps each round's
+ * intervals, features and weights for tfh_proposal_loss_bwd. */
+int32_t tfh_proposal_sample(const tfh_vm* field, const tfh_prop_mlp* mlp,
+                            const tfh_proposal_config* pc, const float* aabb,
+                            const float* origins, const float* directions, int64_t num_rays,
+                            const float* jitter, const float* base_ts, const float* base_steps,
+                            float* edges_out, void* workspace, size_t workspace_bytes,
+                            tfh_stream_t stream);
+/* Device pointers, inside a workspace filled by tfh_proposal_sample, to round
+ * `round`'s interval edges (R, n_round + 1) and proposal weights (R, n_round)
+ * (for inspection, e.g. weight histograms). No kernel runs. */
+int32_t tfh_proposal_round(const tfh_proposal_config* pc, const tfh_vm* field, int64_t num_rays,
+                           int32_t round, const void* workspace, size_t workspace_bytes,
+                           const float** edges, const float** weights);
+/* Interval loss of every round against the field's weights field_w (R, n)
+ * on edges (R, n + 1) (the edges_out of tfh_proposal_sample, n =
+ * samples_out): adds the loss to *loss_out (device float) and accumulates
+ * the proposal gradients. Must follow tfh_proposal_sample with the same
+ * arguments and workspace. */
+int32_t tfh_proposal_loss_bwd(const tfh_vm* field, const tfh_prop_mlp* mlp,
+                              const tfh_proposal_config* pc, int64_t num_rays, const float* edges,
+                              const float* field_w, const tfh_vm_grad* field_grad,
+                              const tfh_prop_mlp_grad* mlp_grad, float* loss_out, void* workspace,
+                              size_t workspace_bytes, tfh_stream_t stream);
+
+/* Fused training step with the proposal sampler: proposal rounds -> render
+ * (RGB, TFH_SAMPLING_EDGES on the proposal's intervals) -> MSE -> backward
+ * -> interval loss backward -> Adam on both parameter sets. cfg must have
+ * sampling = TFH_SAMPLING_EDGES and density_samples_per_ray =
+ * pc->samples_out. jitter: NULL, or the proposal's (R, samples_round0 +
+ * samples_round1) fractions followed by the field's (R, N) fractions;
+ * base_ts / base_steps: the proposal's round-0 schedule (contraction only).
+ * The field's buffers are as tfh_train_step's; prop_buffers holds the
+ * proposal factors (first tensor_count floats, lr_tensor) then the proposal
+ * MLP (lr_mlp). Both gradients must be zero on entry and are left zero.
+ * mse_out and prop_loss_out: device floats, overwritten. */
+size_t tfh_train_step_proposal_workspace_size(const tfh_render_config* cfg, const tfh_mlp* mlp,
+                                              int64_t appearance_channels,
+                                              const tfh_proposal_config* pc,
+                                              const tfh_vm* field, int64_t num_rays);
+int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
+                                const tfh_mlp* mlp, const tfh_render_config* cfg,
+                                const tfh_vm* field, const tfh_prop_mlp* prop_mlp,
+                                const tfh_proposal_config* pc, const float* aabb,
+                                const float* origins, const float* directions,
+                                const uint32_t* camera_indices, const float* colors,
+                                int64_t num_rays, const float* jitter, const float* base_ts,
+                                const float* base_steps, const float* gumbel,
+                                const tfh_vm_grad* density_grad,
+                                const tfh_vm_grad* appearance_grad, const tfh_mlp_grad* mlp_grad,
+                                const tfh_vm_grad* field_grad,
+                                const tfh_prop_mlp_grad* prop_mlp_grad,
+                                const tfh_train_buffers* buffers,
+                                const tfh_train_buffers* prop_buffers, float lr_tensor,
+                                float lr_mlp, float decay, int64_t adam_count, float* mse_out,
+                                float* prop_loss_out, void* workspace, size_t workspace_bytes,
+                                tfh_stream_t stream);
 
 /* ---- synthetic scenes ----------------------------------------------------- */
 /* Ground truth of a procedural scene (replaces the image datasets of data.py):

@@ -141,6 +141,34 @@ def sample_contracted(origins, directions, samples_per_ray, near, far, jitter,
     return np.moveaxis(pts, -1, 0).astype(dtype), ts.astype(dtype), steps.astype(dtype)
 
 
+def contract_points(pts, dtype=np.float64):
+    """L-infinity contraction p -> (2 - 1/|p|_inf) p/|p|_inf outside the unit cube,
+    render.py:168-173. pts (..., 3)."""
+    pts = _c(pts, dtype)
+    norm = np.abs(pts).max(axis=-1, keepdims=True)                   # render.py:170
+    with np.errstate(divide="ignore", invalid="ignore"):
+        contracted = (dtype(2.0) - dtype(1.0) / norm) * pts / norm
+    return np.where(norm <= 1.0, pts, contracted).astype(dtype)      # render.py:171
+
+
+def sample_from_edges(origins, directions, edges, jitter, contraction, dtype=np.float64):
+    """Samples of per-ray intervals (no reference counterpart; the interface a
+    proposal sampler hands to the field): t_i = e_i + u_i (e_{i+1} - e_i), step
+    size e_{i+1} - e_i, u = jitter (R, N) or 0.5. The point is contracted like
+    render.py:168-173 when ``contraction``.
+    Returns points (3, R, N), ts (R, N), step_sizes (R, N)."""
+    o = _c(origins, dtype)
+    d = _c(directions, dtype)
+    e = _c(edges, dtype)
+    steps = e[:, 1:] - e[:, :-1]
+    u = dtype(0.5) if jitter is None else _c(jitter, dtype)
+    ts = e[:, :-1] + u * steps
+    pts = o[:, None, :] + ts[:, :, None] * d[:, None, :]
+    if contraction:
+        pts = contract_points(pts, dtype)
+    return np.moveaxis(pts, -1, 0).astype(dtype), ts.astype(dtype), steps.astype(dtype)
+
+
 def normalize_points(points, aabb, dtype=np.float64):
     """Map world (or contracted) points to [-1, 1] with the aabb, render.py:192-196."""
     p = _c(points, dtype)
@@ -463,21 +491,29 @@ class RenderConfig:
     density_samples_per_ray: int = 64
     appearance_samples_per_ray: int = 64
     scene_contraction: bool = False
+    sampling: int = 0  # 0: the reference's samplers, 1: per-ray interval edges
+
+
+SAMPLING_SCHEDULE, SAMPLING_EDGES = 0, 1
 
 
 def render_rays(params, aabb, origins, directions, camera_indices, cfg: RenderConfig,
-                shape: MlpShape, jitter, gumbel=None, dtype=np.float64):
+                shape: MlpShape, jitter, gumbel=None, dtype=np.float64, edges=None):
     """render.render_rays, render.py:105-279 with _rgb_from_points (render.py:437-549).
 
     params: dict with 'density_vector' (3,Cd,G), 'density_matrix' (3,Cd,G,G),
     'appearance_vector', 'appearance_matrix' and the MLP entries of
     init_mlp_params. jitter: (N,) bbox mode or (R, N) contraction mode.
+    With cfg.sampling == SAMPLING_EDGES the samples come from ``edges`` (R, N+1)
+    (sample_from_edges) and jitter is (R, N) or None.
     Returns (output, cache): output (R,3) for RGB, (R,) for the depth modes."""
     o = _c(origins, dtype)
     d = _c(directions, dtype)
     r = o.shape[0]
     n = cfg.density_samples_per_ray
-    if cfg.scene_contraction:
+    if cfg.sampling == SAMPLING_EDGES:
+        pts, ts, steps = sample_from_edges(o, d, edges, jitter, cfg.scene_contraction, dtype)
+    elif cfg.scene_contraction:
         pts, ts, steps = sample_contracted(o, d, n, cfg.near, cfg.far, jitter, dtype)
     else:
         pts, ts, steps = sample_points_along_ray_within_bbox(o, d, aabb, n, jitter, dtype)

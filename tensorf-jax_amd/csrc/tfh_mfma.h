@@ -46,6 +46,19 @@ struct GlobalB {
   }
 };
 
+// Transposed view of a row-major FP32 (rows x cols) matrix W as a B operand:
+// B(k, j) = W[j][k], zero outside.
+struct GlobalBT {
+  __amdgpu_buffer_rsrc_t r;
+  int rows, cols;  // of W
+  __device__ __forceinline__ GlobalBT(const float* w, int rows_, int cols_)
+      : r(buffer_rsrc(w, rows_ * cols_ * 4)), rows(rows_), cols(cols_) {}
+  __device__ __forceinline__ float operator()(int k, int j) const {
+    const int off = (j < rows && k < cols) ? (j * cols + k) * 4 : kOOB;
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  }
+};
+
 // BF16 table T[j][k] (stride ld) as a B operand: B(k, j) = T[j][k].
 struct PackedB {
   __amdgpu_buffer_rsrc_t r;
@@ -63,8 +76,8 @@ struct PackedB {
 // LDS (row-major, lda), K <= KMAX. The whole B column strip is loaded into
 // registers first (KMAX/4 floats or KMAX/32 bf16x8 per lane) so its L2
 // latency overlaps; A is read from LDS per k-step.
-template <int NR, int KMAX>
-__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, GlobalB b,
+template <int NR, int KMAX, class BF32>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, BF32 b,
                                           int col0, f32x4 (&acc)[NR]) {
   const int lane = lane_id();
   const int r = lane & 15, kq = lane >> 4;

@@ -16,7 +16,7 @@
 //   k_vm_scatter     appearance factor gradients
 #include <algorithm>
 
-#include "tfh_vm_kernels.h"
+#include "tfh_prop.h"
 
 namespace tfh {
 
@@ -33,104 +33,7 @@ int32_t adam_launch(float* p, float* g, float* m, float* v, int64_t n, int64_t s
                     float lr_hi, float decay, float b1, float b2, float eps, int64_t count,
                     int zero_grad, hipStream_t st);
 
-constexpr int kMaxSamples = 1024;
-
-// ---- sampling -------------------------------------------------------------------
-// ray_segment_from_bounding_box (render.py:399-434) + the stratified step
-// (render.py:369): returns t_min and the step size (t_max - t_min) / N.
-__device__ __forceinline__ void bbox_segment(float ox, float oy, float oz, float dx, float dy,
-                                             float dz, const float* __restrict__ aabb, int n,
-                                             float& t0, float& step) {
-  const float o[3] = {ox, oy, oz};
-  const float d[3] = {dx, dy, dz};
-  float tmin_axis_max = -INFINITY, tmax_axis_min = INFINITY;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float den = d[a] + 1e-8f;                 // render.py:413-415
-    const float t_lo = (aabb[a] - o[a]) / den;
-    const float t_hi = (aabb[3 + a] - o[a]) / den;
-    tmin_axis_max = fmaxf(tmin_axis_max, fminf(t_lo, t_hi));  // render.py:418
-    tmax_axis_min = fminf(tmax_axis_min, fmaxf(t_lo, t_hi));  // render.py:419
-  }
-  const float t_min = fmaxf(0.0f, tmin_axis_max);   // render.py:423
-  const float t_max = tmax_axis_min;                // render.py:424
-  const float t_max_clipped = fmaxf(t_max, t_min + 1e-3f);  // render.py:425
-  const bool valid = t_min < t_max;                 // render.py:429
-  t0 = valid ? t_min : 0.0f;                        // render.py:432-433
-  const float t1 = valid ? t_max_clipped : 1e-3f;
-  step = (t1 - t0) / (float)n;                      // render.py:369
-}
-
-struct RayIn {
-  const float* o;
-  const float* d;
-  const float* aabb;  // (2,3) = 6 floats
-  int64_t R;
-  int N, k, mode, contraction;
-  const float* jitter;      // (N) bbox, (R,N) contraction, or null
-  const float* base_ts;     // (N) contraction schedule
-  const float* base_steps;  // (N)
-  const float* gumbel;      // (N) or null
-};
-
-// Sample i of ray r: distance t, step size delta and the world (or
-// contracted) point (render.py:122-186).
-__device__ __forceinline__ void sample_point(const RayIn& in, int64_t r, int i, float ox, float oy,
-                                             float oz, float dx, float dy, float dz, float t0,
-                                             float step, float& t, float& dlt, float& px,
-                                             float& py, float& pz) {
-  if (!in.contraction) {
-    const float u = in.jitter ? in.jitter[i] : 0.0f;
-    t = ((float)i + u) * step;                     // render.py:372-380
-    t = t0 + t;                                    // render.py:381
-    dlt = step;                                    // render.py:186
-    px = ox + dx * t;                              // render.py:384-386
-    py = oy + dy * t;
-    pz = oz + dz * t;
-  } else {
-    const float bs = in.base_steps[i];
-    const float u = in.jitter ? in.jitter[r * in.N + i] : 0.0f;
-    t = in.base_ts[i] + bs * u;                    // render.py:161
-    dlt = bs;                                      // render.py:154-155
-    px = ox + t * dx;                              // render.py:164-167
-    py = oy + t * dy;
-    pz = oz + t * dz;
-    const float nrm = fmaxf(fmaxf(fabsf(px), fabsf(py)), fabsf(pz));  // render.py:170
-    if (!(nrm <= 1.0f)) {                          // render.py:171
-      const float f = 2.0f - 1.0f / nrm;
-      px = f * px / nrm;
-      py = f * py / nrm;
-      pz = f * pz / nrm;
-    }
-  }
-}
-
-// ((p - a0) / (a1 - a0) - 0.5) * 2 (render.py:193-195)
-__device__ __forceinline__ float normalise(float p, float a0, float a1) {
-  return ((p - a0) / (a1 - a0) - 0.5f) * 2.0f;
-}
-
-// ---- per-ray scans and selection (one wave) ---------------------------------------
-// p_exits E_i = exp(sum_{j<=i} -a_j), p_terminates w_i = (1-exp(-a_i)) E_{i-1}
-// (render.py:321-341), from a (LDS) into e / w (LDS).
-__device__ void wave_segment_probs(const float* s_a, float* s_e, float* s_w, int n) {
-  const int lane = lane_id();
-  float carry = 0.0f;
-  for (int c0 = 0; c0 < n; c0 += 64) {
-    const int i = c0 + lane;
-    const bool act = i < n;
-    const float neg = act ? -s_a[i] : 0.0f;
-    const float incl = wave_incl_scan(neg) + carry;
-    float prev = __shfl_up(incl, 1, 64);
-    if (lane == 0) prev = carry;
-    if (act) {
-      s_e[i] = expf(incl);
-      s_w[i] = (1.0f - expf(neg)) * expf(prev);
-    }
-    carry = __shfl(incl, 63, 64);
-  }
-}
-
+// ---- per-ray selection (one wave) -----------------------------------------------
 // Top-k of the keys (LDS, order-preserving uint32) -> ascending indices
 // into idx_out (global, k entries). Ties: lower index first. Returns the sum
 // of s_w over the selection (wave-uniform).
@@ -197,7 +100,8 @@ __global__ void __launch_bounds__(64) k_density_pass(tfh_vm dvm, RayIn in, Densi
   const float ox = in.o[3 * r], oy = in.o[3 * r + 1], oz = in.o[3 * r + 2];
   const float dx = in.d[3 * r], dy = in.d[3 * r + 1], dz = in.d[3 * r + 2];
   float t0 = 0.f, step = 0.f;
-  if (!in.contraction) bbox_segment(ox, oy, oz, dx, dy, dz, in.aabb, N, t0, step);
+  if (!in.contraction && in.sampling == TFH_SAMPLING_SCHEDULE)
+    bbox_segment(ox, oy, oz, dx, dy, dz, in.aabb, N, t0, step);
   const float a0x = in.aabb[0], a0y = in.aabb[1], a0z = in.aabb[2];
   const float a1x = in.aabb[3], a1y = in.aabb[4], a1z = in.aabb[5];
   const int G = (int)dvm.grid_dim, C = (int)dvm.channels;
@@ -498,7 +402,8 @@ __global__ void k_sample_api(RayIn in, float* __restrict__ points, float* __rest
     const float ox = in.o[3 * r], oy = in.o[3 * r + 1], oz = in.o[3 * r + 2];
     const float dx = in.d[3 * r], dy = in.d[3 * r + 1], dz = in.d[3 * r + 2];
     float t0 = 0.f, step = 0.f;
-    if (!in.contraction) bbox_segment(ox, oy, oz, dx, dy, dz, in.aabb, in.N, t0, step);
+    if (!in.contraction && in.sampling == TFH_SAMPLING_SCHEDULE)
+      bbox_segment(ox, oy, oz, dx, dy, dz, in.aabb, in.N, t0, step);
     float t, dlt, px, py, pz;
     sample_point(in, r, i, ox, oy, oz, dx, dy, dz, t0, step, t, dlt, px, py, pz);
     if (points) {
@@ -573,6 +478,8 @@ static int32_t check_cfg(const tfh_render_config* cfg, const char* who) {
                   cfg->appearance_samples_per_ray <= cfg->density_samples_per_ray,
               "%s: appearance_samples_per_ray %d outside [1, N]", who,
               cfg->appearance_samples_per_ray);
+  TFH_REQUIRE(cfg->sampling == TFH_SAMPLING_SCHEDULE || cfg->sampling == TFH_SAMPLING_EDGES,
+              "%s: unknown sampling %d", who, cfg->sampling);
   TFH_REQUIRE(cfg->mode == TFH_MODE_RGB || cfg->mode == TFH_MODE_DIST_MEDIAN ||
                   cfg->mode == TFH_MODE_DIST_MEAN,
               "%s: unknown mode %d", who, cfg->mode);
@@ -619,6 +526,7 @@ static RayIn make_rayin(const tfh_render_config* cfg, const float* aabb, const f
   in.k = cfg->appearance_samples_per_ray;
   in.mode = cfg->mode;
   in.contraction = cfg->scene_contraction ? 1 : 0;
+  in.sampling = cfg->sampling;
   in.jitter = jitter;
   in.base_ts = base_ts;
   in.base_steps = base_steps;
@@ -689,7 +597,9 @@ static int32_t check_render_args(const tfh_vm* density, const tfh_vm* appearance
   TFH_TRY(check_vm(density, who));
   TFH_REQUIRE(R >= 0 && R < (1ll << 31), "%s: num_rays %lld out of range", who, (long long)R);
   TFH_REQUIRE(aabb && o && d, "%s: null aabb/origins/directions", who);
-  if (cfg->scene_contraction && samples)
+  if (cfg->sampling == TFH_SAMPLING_EDGES && samples)
+    TFH_REQUIRE(base_ts, "%s: interval-edge sampling needs the edges (base_ts)", who);
+  else if (cfg->scene_contraction && samples)
     TFH_REQUIRE(base_ts && base_steps, "%s: contraction needs base_ts and base_steps", who);
   (void)jitter;
   if (cfg->mode == TFH_MODE_RGB) {
@@ -883,6 +793,96 @@ int32_t tfh_train_step(const tfh_vm* density, const tfh_vm* appearance, const tf
   return adam_launch(buffers->params, buffers->grads, buffers->exp_avg, buffers->exp_avg_sq,
                      buffers->total, buffers->tensor_count, lr_tensor, lr_mlp, decay, 0.9f, 0.99f,
                      1e-8f, adam_count, 1, st);
+}
+
+size_t tfh_train_step_proposal_workspace_size(const tfh_render_config* cfg, const tfh_mlp* mlp,
+                                              int64_t appearance_channels,
+                                              const tfh_proposal_config* pc,
+                                              const tfh_vm* field, int64_t num_rays) {
+  if (check_cfg(cfg, "tfh_train_step_proposal_workspace_size") != TFH_OK || num_rays < 0 ||
+      !mlp || check_mlp(mlp, "tfh_train_step_proposal_workspace_size") != TFH_OK ||
+      !pc || !field || check_vm(field, "tfh_train_step_proposal_workspace_size") != TFH_OK)
+    return 0;
+  Carve cv(nullptr, 0);
+  RenderWs ws;
+  carve_render(cv, cfg, mlp, 3 * appearance_channels, num_rays, ws);
+  cv.take<float>(num_rays * (cfg->density_samples_per_ray + 1));
+  return cv.off + prop_workspace_bytes(pc, field, num_rays);
+}
+
+int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
+                                const tfh_mlp* mlp, const tfh_render_config* cfg,
+                                const tfh_vm* field, const tfh_prop_mlp* prop_mlp,
+                                const tfh_proposal_config* pc, const float* aabb,
+                                const float* origins, const float* directions,
+                                const uint32_t* camera_indices, const float* colors,
+                                int64_t num_rays, const float* jitter, const float* base_ts,
+                                const float* base_steps, const float* gumbel,
+                                const tfh_vm_grad* density_grad,
+                                const tfh_vm_grad* appearance_grad, const tfh_mlp_grad* mlp_grad,
+                                const tfh_vm_grad* field_grad,
+                                const tfh_prop_mlp_grad* prop_mlp_grad,
+                                const tfh_train_buffers* buffers,
+                                const tfh_train_buffers* prop_buffers, float lr_tensor,
+                                float lr_mlp, float decay, int64_t adam_count, float* mse_out,
+                                float* prop_loss_out, void* workspace, size_t workspace_bytes,
+                                tfh_stream_t stream) {
+  const char* who = "tfh_train_step_proposal";
+  TFH_TRY(check_render_args(density, appearance, mlp, cfg, aabb, origins, directions, num_rays,
+                            nullptr, nullptr, nullptr, who, false));
+  TFH_TRY(check_prop_args(field, prop_mlp, pc, num_rays, who));
+  TFH_REQUIRE(cfg->mode == TFH_MODE_RGB, "%s: the training loss is on RGB", who);
+  TFH_REQUIRE(cfg->sampling == TFH_SAMPLING_EDGES, "%s: cfg->sampling must be EDGES", who);
+  TFH_REQUIRE(cfg->density_samples_per_ray == pc->samples_out,
+              "%s: density_samples_per_ray %d != samples_out %d", who,
+              cfg->density_samples_per_ray, pc->samples_out);
+  TFH_REQUIRE(cfg->scene_contraction == pc->scene_contraction,
+              "%s: scene_contraction differs between the configs", who);
+  TFH_REQUIRE(num_rays >= 1, "%s: empty batch", who);
+  TFH_REQUIRE(colors && mse_out && prop_loss_out && buffers && prop_buffers && density_grad &&
+                  appearance_grad && mlp_grad && field_grad && field_grad->plane &&
+                  field_grad->line && prop_mlp_grad,
+              "%s: null argument", who);
+  if (pc->scene_contraction)
+    TFH_REQUIRE(base_ts && base_steps, "%s: contraction needs the round-0 schedule", who);
+  for (const tfh_train_buffers* b : {buffers, prop_buffers})
+    TFH_REQUIRE(b->params && b->grads && b->exp_avg && b->exp_avg_sq && b->total > 0 &&
+                    b->tensor_count >= 0 && b->tensor_count <= b->total,
+                "%s: bad parameter buffers", who);
+  TFH_REQUIRE(adam_count >= 1, "%s: adam_count must be >= 1", who);
+  TFH_TRY(require_device());
+  hipStream_t st = as_stream(stream);
+  const int64_t R = num_rays;
+  const int N = cfg->density_samples_per_ray;
+  Carve cv(workspace, workspace_bytes);
+  RenderWs ws;
+  carve_render(cv, cfg, mlp, 3 * appearance->channels, R, ws);
+  float* edges = cv.take<float>(R * (N + 1));
+  PropWs pws;
+  carve_prop(cv, pc, field, prop_mlp, R, pws);
+  if (!cv.ok)
+    return fail(TFH_ERR_WORKSPACE, "%s: workspace of %zu bytes, need %zu", who, workspace_bytes,
+                cv.off);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mse_out, (int64_t)1, 0.0f);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, prop_loss_out, (int64_t)1, 0.0f);
+  TFH_TRY(check_launch("train step: clear losses"));
+  const int prop_samples = pc->samples_round0 + pc->samples_round1;
+  TFH_TRY(prop_sample(field, prop_mlp, pc, aabb, origins, directions, R, jitter, base_ts,
+                      base_steps, edges, pws, st));
+  const float* fjit = jitter ? jitter + R * prop_samples : nullptr;
+  RayIn in = make_rayin(cfg, aabb, origins, directions, R, fjit, edges, nullptr, gumbel);
+  TFH_TRY(render_forward(density, appearance, mlp, cfg, in, camera_indices, ws.dc, colors, mse_out,
+                         ws, st));
+  TFH_TRY(render_backward(density, appearance, mlp, cfg, R, directions, camera_indices, ws.gout,
+                          density_grad, appearance_grad, mlp_grad, ws, st));
+  TFH_TRY(prop_loss_bwd(field, prop_mlp, pc, R, edges, ws.d.w, field_grad, prop_mlp_grad,
+                        prop_loss_out, pws, st));
+  TFH_TRY(adam_launch(buffers->params, buffers->grads, buffers->exp_avg, buffers->exp_avg_sq,
+                      buffers->total, buffers->tensor_count, lr_tensor, lr_mlp, decay, 0.9f,
+                      0.99f, 1e-8f, adam_count, 1, st));
+  return adam_launch(prop_buffers->params, prop_buffers->grads, prop_buffers->exp_avg,
+                     prop_buffers->exp_avg_sq, prop_buffers->total, prop_buffers->tensor_count,
+                     lr_tensor, lr_mlp, decay, 0.9f, 0.99f, 1e-8f, adam_count, 1, st);
 }
 
 }  // extern "C"

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libtensorf_hip.so")
 TFH_OK = 0
 MODE_RGB, MODE_DIST_MEDIAN, MODE_DIST_MEAN = 0, 1, 2
 PREC_FP32, PREC_BF16 = 0, 1
+SAMPLING_SCHEDULE, SAMPLING_EDGES = 0, 1
 
 
 class TfhError(RuntimeError):
@@ -59,7 +60,20 @@ class MlpGrad(ctypes.Structure):
 class RenderConfig(ctypes.Structure):
     _fields_ = [("near_plane", c_f), ("far_plane", c_f), ("mode", c_i32),
                 ("density_samples_per_ray", c_i32), ("appearance_samples_per_ray", c_i32),
-                ("scene_contraction", c_i32), ("mlp_precision", c_i32), ("reserved", c_i32)]
+                ("scene_contraction", c_i32), ("mlp_precision", c_i32), ("sampling", c_i32)]
+
+
+class PropMlp(ctypes.Structure):
+    _fields_ = [(n, c_p) for n in ("w1", "b1", "w2", "b2", "w3", "b3")] + [("feat_dim", c_i64)]
+
+
+class PropMlpGrad(ctypes.Structure):
+    _fields_ = [(n, c_p) for n in ("w1", "b1", "w2", "b2", "w3", "b3")]
+
+
+class ProposalConfig(ctypes.Structure):
+    _fields_ = [("samples_round0", c_i32), ("samples_round1", c_i32), ("samples_out", c_i32),
+                ("scene_contraction", c_i32), ("histogram_pad", c_f), ("loss_weight", c_f)]
 
 
 class TrainBuffers(ctypes.Structure):
@@ -81,6 +95,8 @@ _SIGS = {
     "tfh_sample_contracted": (c_i32, [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p,
                                       c_p]),
     "tfh_segment_probabilities": (c_i32, [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p]),
+    "tfh_segment_probabilities_bwd": (c_i32, [c_p, c_p, c_i64, c_i64, c_p, c_p, c_p, c_p, c_p,
+                                              c_p]),
     "tfh_topk_select": (c_i32, [c_p, c_i64, c_i64, c_i64, c_p, c_p, c_p]),
     "tfh_feature_mlp_fwd_workspace_size": (c_sz, [P(Mlp), c_i32]),
     "tfh_feature_mlp_fwd": (c_i32, [P(Mlp), c_p, c_p, c_p, c_i64, c_p, c_i32, c_p, c_sz, c_p]),
@@ -97,6 +113,27 @@ _SIGS = {
     "tfh_train_step": (c_i32, [P(Vm), P(Vm), P(Mlp), P(RenderConfig), c_p, c_p, c_p, c_p, c_p,
                                c_i64, c_p, c_p, c_p, c_p, P(VmGrad), P(VmGrad), P(MlpGrad),
                                P(TrainBuffers), c_f, c_f, c_f, c_i64, c_p, c_p, c_sz, c_p]),
+    "tfh_prop_mlp_fwd": (c_i32, [P(PropMlp), c_p, c_i64, c_p, c_p, c_p]),
+    "tfh_prop_mlp_bwd_workspace_size": (c_sz, [P(PropMlp), c_i64]),
+    "tfh_prop_mlp_bwd": (c_i32, [P(PropMlp), c_p, c_i64, c_p, c_p, P(PropMlpGrad), c_p, c_sz,
+                                 c_p]),
+    "tfh_proposal_resample": (c_i32, [c_p, c_p, c_i64, c_i64, c_i64, c_f, c_p, c_p]),
+    "tfh_interval_loss": (c_i32, [c_p, c_p, c_i64, c_p, c_p, c_i64, c_i64, c_f, c_p, c_p, c_p]),
+    "tfh_proposal_workspace_size": (c_sz, [P(ProposalConfig), P(Vm), c_i64]),
+    "tfh_proposal_sample": (c_i32, [P(Vm), P(PropMlp), P(ProposalConfig), c_p, c_p, c_p, c_i64,
+                                    c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    "tfh_proposal_round": (c_i32, [P(ProposalConfig), P(Vm), c_i64, c_i32, c_p, c_sz, P(c_p),
+                                   P(c_p)]),
+    "tfh_proposal_loss_bwd": (c_i32, [P(Vm), P(PropMlp), P(ProposalConfig), c_i64, c_p, c_p,
+                                      P(VmGrad), P(PropMlpGrad), c_p, c_p, c_sz, c_p]),
+    "tfh_train_step_proposal_workspace_size": (c_sz, [P(RenderConfig), P(Mlp), c_i64,
+                                                      P(ProposalConfig), P(Vm), c_i64]),
+    "tfh_train_step_proposal": (c_i32, [P(Vm), P(Vm), P(Mlp), P(RenderConfig), P(Vm),
+                                        P(PropMlp), P(ProposalConfig), c_p, c_p, c_p, c_p, c_p,
+                                        c_i64, c_p, c_p, c_p, c_p, P(VmGrad), P(VmGrad),
+                                        P(MlpGrad), P(VmGrad), P(PropMlpGrad), P(TrainBuffers),
+                                        P(TrainBuffers), c_f, c_f, c_f, c_i64, c_p, c_p, c_p,
+                                        c_sz, c_p]),
     "tfh_uniform": (c_i32, [ctypes.c_uint64, ctypes.c_uint64, c_i64, c_p, c_p]),
     "tfh_blob_scene_render": (c_i32, [c_p, c_i32, c_p, c_p, c_i64, ctypes.c_double,
                                       ctypes.c_double, c_i32, c_i32, c_p, c_p, c_p]),

@@ -18,7 +18,7 @@ from typing import Dict, Optional, Tuple
 import numpy as np
 import torch
 
-from . import _lib, cameras, networks, tensor_vm
+from . import _lib, cameras, networks, proposal, tensor_vm
 
 
 class RenderMode(enum.Enum):
@@ -37,6 +37,10 @@ class RenderConfig:
     density_samples_per_ray: int
     appearance_samples_per_ray: int
     appearance_sampling: str = "topk"  # "topk" (deterministic) or "gumbel"
+    # Learned proposal sampler (no reference counterpart, render.py:126): when
+    # set, density_samples_per_ray must equal proposal.samples_out and
+    # LearnableParams.proposal_field holds its parameters.
+    proposal: Optional[proposal.ProposalConfig] = None
 
 
 @dataclasses.dataclass
@@ -46,6 +50,7 @@ class LearnableParams:
     appearance_tensor: tensor_vm.TensorVM
     density_tensor: tensor_vm.TensorVM
     scene_contraction: bool
+    proposal_field: Optional[proposal.ProposalField] = None
 
 
 @dataclasses.dataclass
@@ -76,7 +81,7 @@ def _noise(prng_key, ray_count: int, config: RenderConfig, contraction: bool, de
     n = config.density_samples_per_ray
     if prng_key is None:
         jitter = None
-    elif contraction:
+    elif contraction or config.proposal is not None:
         jitter = uniform(_seed(prng_key, 1), ray_count * n, device).view(ray_count, n)
     else:
         jitter = uniform(_seed(prng_key, 1), n, device)
@@ -119,10 +124,19 @@ def _schedule_tensors(n, near, far, device):
 
 # ---- C structs ------------------------------------------------------------------------
 def c_config(config: RenderConfig, contraction: bool, precision: int) -> _lib.RenderConfig:
+    sampling = _lib.SAMPLING_EDGES if config.proposal is not None else _lib.SAMPLING_SCHEDULE
     return _lib.RenderConfig(float(config.near), float(config.far), config.mode.value,
                              int(config.density_samples_per_ray),
                              int(config.appearance_samples_per_ray), int(bool(contraction)),
-                             int(precision), 0)
+                             int(precision), sampling)
+
+
+def proposal_noise(prng_key, ray_count: int, pc: proposal.ProposalConfig, device):
+    """(R, n0 + n1) fractions placing the proposal rounds' samples, or None."""
+    if prng_key is None:
+        return None
+    m = pc.jitter_count()
+    return uniform(_seed(prng_key, 3), ray_count * m, device).view(ray_count, m)
 
 
 class _RenderFn(torch.autograd.Function):
@@ -144,7 +158,8 @@ class _RenderEnv:
     """Holds the structs, noise and workspace of one render_rays call."""
 
     def __init__(self, mlp: networks.FeatureMlp, lp: LearnableParams, aabb, rays, prng_key,
-                 config: RenderConfig, workspace: Optional[torch.Tensor] = None):
+                 config: RenderConfig, workspace: Optional[torch.Tensor] = None,
+                 sample_proposal: bool = True, alloc_workspace: bool = True):
         self.mlp, self.lp, self.config = mlp, lp, config
         self.device = rays.origins.device
         self.aabb = aabb.to(device=self.device, dtype=torch.float32).contiguous()
@@ -156,7 +171,24 @@ class _RenderEnv:
         self.jitter, self.gumbel = _noise(prng_key, self.r, config, self.contraction, self.device)
         n = config.density_samples_per_ray
         self.base_ts = self.base_steps = None
-        if self.contraction:
+        self.sampler = None
+        if config.proposal is not None:
+            pc = config.proposal
+            if lp.proposal_field is None:
+                raise ValueError("config.proposal is set but learnable_params has no proposal_field")
+            if pc.samples_out != n:
+                raise ValueError("density_samples_per_ray must equal proposal.samples_out")
+            self.prop_base_ts = self.prop_base_steps = None
+            if self.contraction:
+                self.prop_base_ts, self.prop_base_steps = _schedule_tensors(
+                    pc.samples_round0, config.near, config.far, self.device)
+            self.prop_jitter = proposal_noise(prng_key, self.r, pc, self.device)
+            if sample_proposal:
+                self.sampler = proposal.Sampler(lp.proposal_field, pc, self.aabb, self.o, self.d,
+                                                self.prop_jitter, self.prop_base_ts,
+                                                self.prop_base_steps, self.contraction)
+                self.base_ts = self.sampler.edges
+        elif self.contraction:
             self.base_ts, self.base_steps = _schedule_tensors(n, config.near, config.far,
                                                               self.device)
         self.cfg = c_config(config, self.contraction, mlp.precision_code if mlp else 0)
@@ -166,7 +198,9 @@ class _RenderEnv:
         self.cmlp = mlp.c_struct(lp.appearance_mlp_params, 3 * ca) if mlp else None
         ws_bytes = _lib.lib.tfh_render_workspace_size(
             ctypes.byref(self.cfg), ctypes.byref(self.cmlp) if self.cmlp else None, ca, self.r)
-        if workspace is not None and workspace.numel() >= int(ws_bytes):
+        if not alloc_workspace:
+            self.ws = None  # the caller runs a fused entry point with its own workspace
+        elif workspace is not None and workspace.numel() >= int(ws_bytes):
             self.ws = workspace
         else:
             self.ws = torch.empty(max(int(ws_bytes), 1), dtype=torch.uint8, device=self.device)

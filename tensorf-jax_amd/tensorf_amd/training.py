@@ -17,7 +17,7 @@ from typing import Dict, List, Optional, Tuple
 
 import torch
 
-from . import _lib, cameras, networks, render, tensor_vm, utils
+from . import _lib, cameras, networks, proposal, render, tensor_vm, utils
 
 
 @dataclasses.dataclass(frozen=True)
@@ -59,6 +59,9 @@ class TensorfConfig:
     density_samples_per_ray: Optional[int] = None
     appearance_samples_per_ray: Optional[int] = None
     mlp_precision: str = "fp32"
+    # Learned proposal sampler (no reference counterpart): when set, the field
+    # takes proposal.samples_out samples per ray on the proposal's intervals.
+    proposal: Optional[proposal.ProposalConfig] = None
 
 
 def lr_decay_coeff(step: int, config: TensorfConfig) -> float:
@@ -74,7 +77,12 @@ def lr_decay_coeff(step: int, config: TensorfConfig) -> float:
 
 
 def sample_counts(config: TensorfConfig, grid_dim: int) -> Tuple[int, int]:
-    """training.py:115-118, unless the config pins the counts."""
+    """training.py:115-118, unless the config pins the counts (or a proposal
+    sampler fixes N)."""
+    if config.proposal is not None:
+        n = config.proposal.samples_out
+        k = config.appearance_samples_per_ray or n
+        return n, max(1, min(k, n))
     n = config.density_samples_per_ray or int(
         math.sqrt(3 * grid_dim ** 2) * config.train_ray_sample_multiplier)
     k = config.appearance_samples_per_ray or int(0.15 * n)
@@ -126,6 +134,8 @@ class TrainState:
         self.adam_count = adam_count
         self._ws = None
         self._mse = torch.zeros(1, dtype=torch.float32, device=aabb.device)
+        self._prop_loss = torch.zeros(1, dtype=torch.float32, device=aabb.device)
+        self.prop_flat: Optional[FlatParams] = None
 
     # ---- construction -----------------------------------------------------------
     @staticmethod
@@ -145,7 +155,18 @@ class TrainState:
                             dtype=torch.float32, device=device)
         state = TrainState(config, mlp, None, aabb, int(prng_key))
         state._load(den, app, mparams)
+        if config.proposal is not None:
+            state._load_proposal(proposal.ProposalField.initialize(config.proposal, gen, device))
         return state
+
+    def _load_proposal(self, field: proposal.ProposalField):
+        tensors = field.leaves()
+        names = ["prop_plane", "prop_line", *proposal.MLP_KEYS]
+        flat = FlatParams([(n, tuple(t.shape)) for n, t in zip(names, tensors)], 2,
+                          self.aabb.device)
+        for i, t in enumerate(tensors):
+            flat.view(flat.params, i).copy_(t)
+        self.prop_flat = flat
 
     def _load(self, den: tensor_vm.TensorVM, app: tensor_vm.TensorVM, mparams: Dict,
               moments=None):
@@ -180,19 +201,31 @@ class TrainState:
             p["Embed_0"] = {"embedding": f.view(buf, 11)}
         return {"params": p}
 
+    def _proposal_field(self, buf) -> Optional[proposal.ProposalField]:
+        f = self.prop_flat
+        if f is None:
+            return None
+        vm = tensor_vm.TensorVM(tensor_vm.TensorVMSingle(f.view(buf, 0), f.view(buf, 1)))
+        return proposal.ProposalField(vm, {k: f.view(buf, 2 + i)
+                                           for i, k in enumerate(proposal.MLP_KEYS)})
+
     @property
     def learnable_params(self) -> render.LearnableParams:
-        return render.LearnableParams(self._mlp_params(self.flat.params),
-                                      self._vm(self.flat.params, 2), self._vm(self.flat.params, 0),
-                                      self.config.scene_contraction)
+        return render.LearnableParams(
+            self._mlp_params(self.flat.params), self._vm(self.flat.params, 2),
+            self._vm(self.flat.params, 0), self.config.scene_contraction,
+            self._proposal_field(self.prop_flat.params) if self.prop_flat else None)
 
     def render_config(self, mode=render.RenderMode.RGB) -> render.RenderConfig:
         n, k = sample_counts(self.config, self.flat.shapes[0][1])
-        return render.RenderConfig(self.config.render_near, self.config.render_far, mode, n, k)
+        return render.RenderConfig(self.config.render_near, self.config.render_far, mode, n, k,
+                                   proposal=self.config.proposal)
 
     # ---- training ------------------------------------------------------------------
     def training_step(self, minibatch: cameras.RenderedRays) -> Tuple["TrainState", Dict]:
         """training.py:101-205 as one library call. Logs hold device scalars."""
+        if self.config.proposal is not None:
+            return self._training_step_proposal(minibatch)
         rays = minibatch.rays_wrt_world.reshape_flat()
         colors = minibatch.colors.reshape(-1, 3).to(torch.float32).contiguous()
         lp = self.learnable_params
@@ -221,6 +254,66 @@ class TrainState:
                   _lib.ptr(env.ws), env.ws.numel(), _lib.stream())
         self.step += 1
         logs = {"mse": self._mse, "lr_tensor": decay * self.config.optimizer.lr_init_tensor,
+                "lr_mlp": decay * self.config.optimizer.lr_init_mlp}
+        return self, logs
+
+    def _grad_structs(self):
+        grads_lp = (self._vm(self.flat.grads, 0), self._vm(self.flat.grads, 2),
+                    self._mlp_params(self.flat.grads))
+        dg = _lib.VmGrad(_lib.ptr(grads_lp[0].stacked_single_vm.plane),
+                         _lib.ptr(grads_lp[0].stacked_single_vm.line))
+        ag = _lib.VmGrad(_lib.ptr(grads_lp[1].stacked_single_vm.plane),
+                         _lib.ptr(grads_lp[1].stacked_single_vm.line))
+        leaves = networks.FeatureMlp.flat_leaves(grads_lp[2])
+        mg = _lib.MlpGrad(*[_lib.ptr(t) for t in leaves], *([None] * (8 - len(leaves))))
+        return dg, ag, mg
+
+    def _training_step_proposal(self, minibatch: cameras.RenderedRays):
+        """Training step with the proposal sampler as one library call
+        (tfh_train_step_proposal): proposal rounds -> render on the proposal's
+        intervals -> MSE and interval-loss backward -> Adam on both sets."""
+        rays = minibatch.rays_wrt_world.reshape_flat()
+        colors = minibatch.colors.reshape(-1, 3).to(torch.float32).contiguous()
+        lp = self.learnable_params
+        cfg = self.render_config()
+        pc = self.config.proposal
+        env = render._RenderEnv(self.appearance_mlp, lp, self.aabb, rays, None, cfg,
+                                sample_proposal=False, alloc_workspace=False)
+        r, n = env.r, cfg.density_samples_per_ray
+        seed = render._seed(self.prng_key, self.step)
+        jitter = render.uniform(seed, r * (pc.jitter_count() + n), env.device)
+        field = lp.proposal_field
+        pvm, pmlp = field.c_vm(), field.c_mlp()
+        cpc = pc.c_struct(env.contraction)
+        ca = lp.appearance_tensor.stacked_single_vm.channel_dim()
+        need = int(_lib.lib.tfh_train_step_proposal_workspace_size(
+            ctypes.byref(env.cfg), ctypes.byref(env.cmlp), ca, ctypes.byref(cpc),
+            ctypes.byref(pvm), r))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=env.device)
+        dg, ag, mg = self._grad_structs()
+        pgrad = self._proposal_field(self.prop_flat.grads)
+        pg_vm = _lib.VmGrad(_lib.ptr(pgrad.tensor.stacked_single_vm.plane),
+                            _lib.ptr(pgrad.tensor.stacked_single_vm.line))
+        pg_mlp = proposal.mlp_grad_struct(pgrad.mlp)
+        bufs = self.flat.c_struct()
+        pbufs = self.prop_flat.c_struct()
+        decay = lr_decay_coeff(self.step, self.config)
+        self.adam_count += 1
+        self._keep = (colors, jitter, env)
+        _lib.call("tfh_train_step_proposal", ctypes.byref(env.dvm), ctypes.byref(env.avm),
+                  ctypes.byref(env.cmlp), ctypes.byref(env.cfg), ctypes.byref(pvm),
+                  ctypes.byref(pmlp), ctypes.byref(cpc), _lib.ptr(env.aabb), _lib.ptr(env.o),
+                  _lib.ptr(env.d), _lib.ptr(env.cams), _lib.ptr(colors), r, _lib.ptr(jitter),
+                  _lib.ptr(env.prop_base_ts), _lib.ptr(env.prop_base_steps), None,
+                  ctypes.byref(dg), ctypes.byref(ag), ctypes.byref(mg), ctypes.byref(pg_vm),
+                  ctypes.byref(pg_mlp), ctypes.byref(bufs), ctypes.byref(pbufs),
+                  self.config.optimizer.lr_init_tensor, self.config.optimizer.lr_init_mlp, decay,
+                  self.adam_count, _lib.ptr(self._mse), _lib.ptr(self._prop_loss),
+                  _lib.ptr(self._ws), self._ws.numel(), _lib.stream())
+        self.step += 1
+        logs = {"mse": self._mse, "prop_loss": self._prop_loss,
+                "lr_tensor": decay * self.config.optimizer.lr_init_tensor,
                 "lr_mlp": decay * self.config.optimizer.lr_init_mlp}
         return self, logs
 

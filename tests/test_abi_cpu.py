@@ -40,7 +40,7 @@ def test_every_declared_symbol_is_exported(lib):
 
 def test_version_and_errors_without_device(lib):
     import torch
-    assert lib.lib.tfh_abi_version() == 1
+    assert lib.lib.tfh_abi_version() == 2
     if torch.cuda.is_available():
         pytest.skip("a GPU is visible")
     assert lib.device_count() == 0
