@@ -406,11 +406,15 @@ int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
  * background, emission-absorption midpoint quadrature in FP64 with
  * samples_per_ray samples over [t_near, t_far] (uniform in t, or with
  * contracted_far != 0 uniform in t up to t=1 and uniform in 1/t beyond).
- * rgb (R,3); opacity (R) may be NULL. No counterpart in the reference. */
+ * rgb (R,3); opacity (R) and median_depth (R) may be NULL. median_depth is
+ * the distance at which the transmittance crosses 1/2 (density constant
+ * within a quadrature step), +inf when it never does. No counterpart in the
+ * reference. */
 int32_t tfh_blob_scene_render(const double* blobs, int32_t num_blobs, const float* origins,
                               const float* directions, int64_t num_rays, double t_near,
                               double t_far, int32_t samples_per_ray, int32_t contracted_far,
-                              float* rgb, float* opacity, tfh_stream_t stream);
+                              float* rgb, float* opacity, float* median_depth,
+                              tfh_stream_t stream);
 
 /* ---- stage timing --------------------------------------------------------- */
 /* Per-stage timing with HIP events recorded on the launching stream around

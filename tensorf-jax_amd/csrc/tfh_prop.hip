@@ -11,8 +11,8 @@
 //                    padded histogram CDF, inverse-CDF edges of the next round
 // Backward (interval loss of mip-NeRF 360 against the field's weights):
 //   k_prop_loss_bwd  one wave per ray: bounds by binary search over the edges,
-//                    dL/dw of the round by two prefix sums, segment VJP ->
-//                    dL/dz
+//                    dL/dw of the round by sums over the overlapping index
+//                    ranges, segment VJP -> dL/dz
 //   prop_mlp_bwd     persistent MLP backward -> dL/dfeatures
 //   k_vm_scatter     proposal factor gradients
 #include <algorithm>
@@ -69,11 +69,13 @@ __device__ inline void wave_resample(const float* s_edge, const float* s_w, int 
 //   bound_i = sum of pw_j over the j with pe_{j+1} > fe_i and pe_j < fe_{i+1}
 //   loss = scale * sum_i max(0, fw_i - bound_i)^2 / (fw_i + eps)
 // and s_G[j] = dloss/dpw_j. Returns the loss (wave-uniform). Scratch: s_W
-// (n+1), s_g (nf), s_P (nf+1).
+// (n+1), s_g (nf). The field intervals overlapping proposal interval j are
+// the index range [c_hi, c_lo) (both edge arrays are sorted); s_G[j] sums
+// their loss gradients directly (a difference of prefix sums would cancel
+// badly in FP32). Over all j the ranges hold about nf + n entries.
 __device__ inline float wave_interval_loss(const float* s_fe, const float* __restrict__ fw, int nf,
                                            const float* s_pe, const float* s_pw, int n,
-                                           float scale, float* s_W, float* s_g, float* s_P,
-                                           float* s_G) {
+                                           float scale, float* s_W, float* s_g, float* s_G) {
   const int lane = lane_id();
   wave_excl_prefix(s_pw, s_W, n);
   __syncthreads();
@@ -89,12 +91,12 @@ __device__ inline float wave_interval_loss(const float* s_fe, const float* __res
     s_g[i] = -2.0f * dlt * inv * scale;
   }
   __syncthreads();
-  wave_excl_prefix(s_g, s_P, nf);
-  __syncthreads();
   for (int j = lane; j < n; j += 64) {
     const int c_lo = lower_bound(s_fe, nf, s_pe[j + 1]);  // i with fe_i < pe_{j+1}
     const int c_hi = upper_bound(s_fe + 1, nf, s_pe[j]);  // i with fe_{i+1} <= pe_j
-    s_G[j] = c_lo > c_hi ? s_P[c_lo] - s_P[c_hi] : 0.0f;
+    float acc = 0.0f;
+    for (int i = c_hi; i < c_lo; ++i) acc += s_g[i];
+    s_G[j] = acc;
   }
   __syncthreads();
   return wave_sum(loss) * scale;
@@ -178,6 +180,22 @@ __global__ void __launch_bounds__(64) k_prop_resample(const float* __restrict__ 
   wave_resample(s_edge, s_w, n, n_out, pad, s_a, s_cdf, next_edges + r * (n_out + 1));
 }
 
+// LDS layout (in floats) of k_prop_loss_bwd, shared by the kernel and its launch.
+struct LossLds {
+  int fe, pe, pw, W, g, G, e, da, total;
+  __host__ __device__ LossLds(int nf, int n) {
+    fe = 0;             // nf+1 field edges
+    pe = fe + nf + 1;   // n+1 round edges
+    pw = pe + n + 1;    // n round weights
+    W = pw + n;         // n+1 prefix sums of the round weights
+    g = W + n + 1;      // nf loss gradients of the field intervals
+    G = g + nf;         // n dloss/dw of the round
+    e = G + n;          // n round transmittances
+    da = e + n;         // n dloss/da of the round
+    total = da + n;
+  }
+};
+
 // Interval loss of one round and its backward to dL/dz of the round's samples.
 __global__ void __launch_bounds__(64) k_prop_loss_bwd(
     const float* __restrict__ fedges, const float* __restrict__ fw, int nf,
@@ -186,15 +204,15 @@ __global__ void __launch_bounds__(64) k_prop_loss_bwd(
     float* __restrict__ loss_out) {
   extern __shared__ float sm[];
   const int64_t r = blockIdx.x;
-  float* s_fe = sm;            // nf+1
-  float* s_pe = s_fe + nf + 1; // n+1
-  float* s_pw = s_pe + n + 1;  // n
-  float* s_W = s_pw + n;       // n+1
-  float* s_g = s_W + n + 1;    // nf
-  float* s_P = s_g + nf;       // nf+1
-  float* s_G = s_P + nf + 1;   // n
-  float* s_e = s_G + n;        // n
-  float* s_da = s_e + n;       // n
+  const LossLds L(nf, n);
+  float* s_fe = sm + L.fe;
+  float* s_pe = sm + L.pe;
+  float* s_pw = sm + L.pw;
+  float* s_W = sm + L.W;
+  float* s_g = sm + L.g;
+  float* s_G = sm + L.G;
+  float* s_e = sm + L.e;
+  float* s_da = sm + L.da;
   for (int i = threadIdx.x; i <= nf; i += 64) s_fe[i] = fedges[r * (nf + 1) + i];
   for (int j = threadIdx.x; j <= n; j += 64) s_pe[j] = pedges[r * (n + 1) + j];
   for (int j = threadIdx.x; j < n; j += 64) {
@@ -203,7 +221,7 @@ __global__ void __launch_bounds__(64) k_prop_loss_bwd(
   }
   __syncthreads();
   const float loss =
-      wave_interval_loss(s_fe, fw + r * nf, nf, s_pe, s_pw, n, scale, s_W, s_g, s_P, s_G);
+      wave_interval_loss(s_fe, fw + r * nf, nf, s_pe, s_pw, n, scale, s_W, s_g, s_G);
   if (threadIdx.x == 0 && loss != 0.0f) unsafeAtomicAdd(loss_out, loss);
   wave_segment_vjp(s_G, nullptr, s_pw, s_e, s_da, n);
   __syncthreads();
@@ -242,14 +260,13 @@ __global__ void __launch_bounds__(64) k_interval_loss_api(
   float* s_pw = s_pe + n + 1;
   float* s_W = s_pw + n;
   float* s_g = s_W + n + 1;
-  float* s_P = s_g + nf;
-  float* s_G = s_P + nf + 1;
+  float* s_G = s_g + nf;
   for (int i = threadIdx.x; i <= nf; i += 64) s_fe[i] = fedges[r * (nf + 1) + i];
   for (int j = threadIdx.x; j <= n; j += 64) s_pe[j] = pedges[r * (n + 1) + j];
   for (int j = threadIdx.x; j < n; j += 64) s_pw[j] = pw[r * n + j];
   __syncthreads();
   const float l =
-      wave_interval_loss(s_fe, fw + r * nf, nf, s_pe, s_pw, n, scale, s_W, s_g, s_P, s_G);
+      wave_interval_loss(s_fe, fw + r * nf, nf, s_pe, s_pw, n, scale, s_W, s_g, s_G);
   if (threadIdx.x == 0) loss[r] = l;
   if (gpw)
     for (int j = threadIdx.x; j < n; j += 64) gpw[r * n + j] = s_G[j];
@@ -372,7 +389,7 @@ int32_t prop_loss_bwd(const tfh_vm* field, const tfh_prop_mlp* mlp,
     ProfScope ps("prop_loss_bwd", st);
     for (int k = 0; k < ws.rounds; ++k) {
       const PropRound& rd = ws.rd[k];
-      const size_t lds = (size_t)(3 * (nf + 1) + 5 * rd.n + 2) * sizeof(float);
+      const size_t lds = (size_t)LossLds(nf, rd.n).total * sizeof(float);
       hipLaunchKernelGGL(k_prop_loss_bwd, dim3((unsigned)R), dim3(64), lds, st, fedges, fw, nf,
                          rd.edges, rd.w, rd.e, rd.sigma, rd.n, scale, rd.dz, loss_out);
       TFH_TRY(check_launch("proposal loss bwd"));
@@ -504,7 +521,7 @@ int32_t tfh_interval_loss(const float* edges, const float* weights, int64_t num_
               "tfh_interval_loss: null pointer");
   TFH_TRY(require_device());
   const int64_t nf = num_intervals, n = num_prop_intervals;
-  const size_t lds = (size_t)(3 * (nf + 1) + 4 * n + 2) * sizeof(float);
+  const size_t lds = (size_t)(2 * (nf + 1) + 4 * n + 2) * sizeof(float);
   hipLaunchKernelGGL(k_interval_loss_api, dim3((unsigned)num_rays), dim3(64), lds,
                      as_stream(stream), edges, weights, (int)nf, prop_edges, prop_weights, (int)n,
                      scale, loss, grad_prop_weights);

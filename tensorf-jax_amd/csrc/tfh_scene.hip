@@ -14,7 +14,8 @@ __global__ void __launch_bounds__(256) k_blob_render(const double* __restrict__ 
                                                      const float* __restrict__ dirs, int64_t R,
                                                      double t_near, double t_far, int n_samples,
                                                      int contracted_far, float* __restrict__ rgb,
-                                                     float* __restrict__ opacity) {
+                                                     float* __restrict__ opacity,
+                                                     float* __restrict__ median_depth) {
   extern __shared__ double sb[];
   for (int i = threadIdx.x; i < nb * 8; i += blockDim.x) sb[i] = blobs[i];
   __syncthreads();
@@ -23,10 +24,12 @@ __global__ void __launch_bounds__(256) k_blob_render(const double* __restrict__ 
   const double ox = origins[3 * r], oy = origins[3 * r + 1], oz = origins[3 * r + 2];
   const double dx = dirs[3 * r], dy = dirs[3 * r + 1], dz = dirs[3 * r + 2];
   double logT = 0.0, cr = 0.0, cg = 0.0, cb = 0.0;
+  double depth = INFINITY;  // where the transmittance crosses 1/2
+  const double log_half = -0.69314718055994530942;
   // midpoint rule; with contracted_far the samples are uniform in 1/t past t=1
   // (unbounded scenes), otherwise uniform in t over [t_near, t_far]
   for (int i = 0; i < n_samples; ++i) {
-    double t, dt;
+    double t, dt, ta;
     const double u0 = (double)i / n_samples, u1 = (double)(i + 1) / n_samples;
     if (contracted_far) {
       // s in [0, 1]: t = t_near + s * (1 - t_near) for s <= 0.5 (linear),
@@ -36,12 +39,14 @@ __global__ void __launch_bounds__(256) k_blob_render(const double* __restrict__ 
         const double q = 2.0 * (s - 0.5);
         return 1.0 / (1.0 + q * (1.0 / t_far - 1.0));
       };
-      const double ta = tmap(u0), tb = tmap(u1);
+      ta = tmap(u0);
+      const double tb = tmap(u1);
       t = 0.5 * (ta + tb);
       dt = tb - ta;
     } else {
       dt = (t_far - t_near) / n_samples;
       t = t_near + (i + 0.5) * dt;
+      ta = t - 0.5 * dt;
     }
     const double px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
     double sig = 0.0, ar = 0.0, ag = 0.0, ab = 0.0;
@@ -60,6 +65,8 @@ __global__ void __launch_bounds__(256) k_blob_render(const double* __restrict__ 
       cr += w * ar / sig;
       cg += w * ag / sig;
       cb += w * ab / sig;
+      // the density is taken constant over the step: log T falls linearly
+      if (logT > log_half && logT - a <= log_half) depth = ta + (logT - log_half) / a * dt;
       logT -= a;
     }
   }
@@ -68,6 +75,7 @@ __global__ void __launch_bounds__(256) k_blob_render(const double* __restrict__ 
   rgb[3 * r + 1] = (float)(cg + T);
   rgb[3 * r + 2] = (float)(cb + T);
   if (opacity) opacity[r] = (float)(1.0 - T);
+  if (median_depth) median_depth[r] = (float)depth;
 }
 
 }  // namespace tfh
@@ -78,7 +86,8 @@ extern "C" int32_t tfh_blob_scene_render(const double* blobs, int32_t num_blobs,
                                          const float* origins, const float* directions,
                                          int64_t num_rays, double t_near, double t_far,
                                          int32_t samples_per_ray, int32_t contracted_far,
-                                         float* rgb, float* opacity, tfh_stream_t stream) {
+                                         float* rgb, float* opacity, float* median_depth,
+                                         tfh_stream_t stream) {
   TFH_REQUIRE(num_blobs >= 1 && num_blobs <= 1024, "tfh_blob_scene_render: num_blobs %d", num_blobs);
   TFH_REQUIRE(num_rays >= 0 && samples_per_ray >= 1, "tfh_blob_scene_render: bad sizes");
   TFH_REQUIRE(t_far > t_near && t_near >= 0.0, "tfh_blob_scene_render: need 0 <= t_near < t_far");
@@ -90,6 +99,6 @@ extern "C" int32_t tfh_blob_scene_render(const double* blobs, int32_t num_blobs,
   hipLaunchKernelGGL(k_blob_render, dim3((unsigned)blocks), dim3(256),
                      (size_t)num_blobs * 8 * sizeof(double), as_stream(stream), blobs, num_blobs,
                      origins, directions, num_rays, t_near, t_far, samples_per_ray,
-                     contracted_far, rgb, opacity);
+                     contracted_far, rgb, opacity, median_depth);
   return check_launch("tfh_blob_scene_render");
 }

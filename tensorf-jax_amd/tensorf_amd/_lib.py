@@ -136,7 +136,7 @@ _SIGS = {
                                         c_sz, c_p]),
     "tfh_uniform": (c_i32, [ctypes.c_uint64, ctypes.c_uint64, c_i64, c_p, c_p]),
     "tfh_blob_scene_render": (c_i32, [c_p, c_i32, c_p, c_p, c_i64, ctypes.c_double,
-                                      ctypes.c_double, c_i32, c_i32, c_p, c_p, c_p]),
+                                      ctypes.c_double, c_i32, c_i32, c_p, c_p, c_p, c_p]),
     "tfh_profile_enable": (c_i32, [c_i32]),
     "tfh_profile_read": (c_i32, [c_i32, c_p, P(ctypes.c_double), P(c_i64), P(c_i32)]),
 }

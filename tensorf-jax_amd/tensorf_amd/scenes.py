@@ -104,14 +104,16 @@ def fisheye_rays(c2w: np.ndarray, width: int, height: int, focal: float, camera_
 
 def render_ground_truth(scene: BlobScene, rays: cameras.Rays3D, samples_per_ray: int = 4096,
                         t_near: Optional[float] = None, t_far: Optional[float] = None,
-                        chunk: int = 1 << 20):
-    """FP64 quadrature of the scene along the rays: (colours (*, 3), opacity (*,))."""
+                        chunk: int = 1 << 20, with_depth: bool = False):
+    """FP64 quadrature of the scene along the rays: (colours (*, 3), opacity (*,))
+    and, with_depth, the median termination depth (*,) (+inf if never reached)."""
     flat = rays.reshape_flat()
     n = flat.origins.shape[0]
     dev = flat.origins.device
     blobs = torch.tensor(scene.blobs, dtype=torch.float64, device=dev).contiguous()
     rgb = torch.empty((n, 3), dtype=torch.float32, device=dev)
     opa = torch.empty((n,), dtype=torch.float32, device=dev)
+    depth = torch.empty((n,), dtype=torch.float32, device=dev) if with_depth else None
     if scene.unbounded:
         tn, tf = (t_near or 0.05), (t_far or 200.0)
     else:
@@ -122,9 +124,12 @@ def render_ground_truth(scene: BlobScene, rays: cameras.Rays3D, samples_per_ray:
         d = flat.directions[s:e].contiguous()
         _lib.call("tfh_blob_scene_render", _lib.ptr(blobs), scene.blobs.shape[0], _lib.ptr(o),
                   _lib.ptr(d), e - s, float(tn), float(tf), int(samples_per_ray),
-                  int(scene.unbounded), _lib.ptr(rgb[s:e]), _lib.ptr(opa[s:e]), _lib.stream())
+                  int(scene.unbounded), _lib.ptr(rgb[s:e]), _lib.ptr(opa[s:e]),
+                  _lib.ptr(depth[s:e]) if with_depth else None, _lib.stream())
         torch.cuda.synchronize()
     shape = rays.get_batch_axes()
+    if with_depth:
+        return rgb.view(*shape, 3), opa.view(*shape), depth.view(*shape)
     return rgb.view(*shape, 3), opa.view(*shape)
 
 

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes
 import dataclasses
+import json
 import math
 import pathlib
 from typing import Dict, List, Optional, Tuple
@@ -339,6 +340,114 @@ class TrainState:
 
     def psnr(self) -> float:
         return utils.psnr_from_mse(self._mse.item())
+
+
+def _config_to_json(config: TensorfConfig) -> str:
+    d = dataclasses.asdict(config)
+    for key in ("run_dir", "dataset_path"):
+        if d[key] is not None:
+            d[key] = str(d[key])
+    return json.dumps(d)
+
+
+def _config_from_json(text: str) -> TensorfConfig:
+    d = json.loads(text)
+    d["optimizer"] = OptimizerConfig(**d["optimizer"])
+    if d.get("proposal") is not None:
+        d["proposal"] = proposal.ProposalConfig(**d["proposal"])
+    for key in ("initial_aabb_min", "initial_aabb_max", "upsamp_iters"):
+        d[key] = tuple(d[key])
+    for key in ("run_dir", "dataset_path"):
+        if d[key] is not None:
+            d[key] = pathlib.Path(d[key])
+    return TensorfConfig(**d)
+
+
+def save_checkpoint(state: TrainState, path) -> None:
+    """Training state as one .npz in the reference's layouts (the fifteen
+    checkpoint of training.py:279-368 / render_360.py:66-74 is not available):
+    TensorVM factors as (3, C, G) vectors and (3, C, G, G) matrices
+    (tensor_vm.py:129-133), Flax-named MLP leaves, Adam moments in the same
+    layouts, step counters, aabb and the config as JSON. Loaded by
+    load_checkpoint; no pickled objects."""
+    import numpy as np
+    out = {"config": np.array(_config_to_json(state.config)),
+           "step": np.array(state.step), "adam_count": np.array(state.adam_count),
+           "prng_key": np.array(state.prng_key), "aabb": state.aabb.cpu().numpy()}
+    f = state.flat
+    for tag, buf in (("params", f.params), ("exp_avg", f.exp_avg), ("exp_avg_sq", f.exp_avg_sq)):
+        for name, i in (("density_tensor", 0), ("appearance_tensor", 2)):
+            vec, mat = state._vm(buf, i).to_reference_layout()
+            out[f"{tag}/{name}/vector"] = vec.cpu().numpy()
+            out[f"{tag}/{name}/matrix"] = mat.cpu().numpy()
+        for layer, leaves in state._mlp_params(buf)["params"].items():
+            for leaf, t in leaves.items():
+                out[f"{tag}/appearance_mlp_params/params/{layer}/{leaf}"] = t.cpu().numpy()
+        if state.prop_flat is not None:
+            field = state._proposal_field({"params": state.prop_flat.params,
+                                           "exp_avg": state.prop_flat.exp_avg,
+                                           "exp_avg_sq": state.prop_flat.exp_avg_sq}[tag])
+            vec, mat = field.tensor.to_reference_layout()
+            out[f"{tag}/proposal_field/vector"] = vec.cpu().numpy()
+            out[f"{tag}/proposal_field/matrix"] = mat.cpu().numpy()
+            for key, t in field.mlp.items():
+                out[f"{tag}/proposal_field/mlp/{key}"] = t.cpu().numpy()
+    np.savez(path, **out)
+
+
+def load_checkpoint(path, device="cuda") -> TrainState:
+    """Inverse of save_checkpoint (numpy.load without pickles)."""
+    import numpy as np
+    z = np.load(path, allow_pickle=False)
+    config = _config_from_json(str(z["config"]))
+
+    def t(key):
+        return torch.tensor(z[key], device=device)
+
+    mlp_names = (("Dense_0", "kernel"), ("Dense_1", "kernel"), ("Dense_1", "bias"),
+                 ("Dense_2", "kernel"), ("Dense_2", "bias"), ("Dense_3", "kernel"),
+                 ("Dense_3", "bias"), ("Embed_0", "embedding"))
+
+    def pieces(tag):
+        den = tensor_vm.TensorVM.from_reference_layout(t(f"{tag}/density_tensor/vector"),
+                                                       t(f"{tag}/density_tensor/matrix"))
+        app = tensor_vm.TensorVM.from_reference_layout(t(f"{tag}/appearance_tensor/vector"),
+                                                       t(f"{tag}/appearance_tensor/matrix"))
+        mp = {}
+        for layer, leaf in mlp_names:
+            key = f"{tag}/appearance_mlp_params/params/{layer}/{leaf}"
+            if key in z:
+                mp.setdefault(layer, {})[leaf] = t(key)
+        return den, app, {"params": mp}
+
+    num_cameras = z["params/appearance_mlp_params/params/Embed_0/embedding"].shape[0] \
+        if "params/appearance_mlp_params/params/Embed_0/embedding" in z else None
+    mlp = networks.FeatureMlp(feature_n_freqs=config.feature_n_freqs,
+                              viewdir_n_freqs=config.viewdir_n_freqs,
+                              num_cameras=num_cameras, precision=config.mlp_precision)
+    aabb = torch.tensor(z["aabb"], dtype=torch.float32, device=device)
+    state = TrainState(config, mlp, None, aabb, int(z["prng_key"]), int(z["step"]),
+                       int(z["adam_count"]))
+    den, app, mp = pieces("params")
+    mom = []
+    for tag in ("exp_avg", "exp_avg_sq"):
+        d2, a2, m2 = pieces(tag)
+        mom.append([d2.stacked_single_vm.plane, d2.stacked_single_vm.line,
+                    a2.stacked_single_vm.plane, a2.stacked_single_vm.line,
+                    *networks.FeatureMlp.flat_leaves(m2)])
+    state._load(den, app, mp, list(zip(mom[0], mom[1])))
+    if "params/proposal_field/vector" in z:
+        fields = {}
+        for tag in ("params", "exp_avg", "exp_avg_sq"):
+            vm = tensor_vm.TensorVM.from_reference_layout(t(f"{tag}/proposal_field/vector"),
+                                                          t(f"{tag}/proposal_field/matrix"))
+            fields[tag] = proposal.ProposalField(
+                vm, {k: t(f"{tag}/proposal_field/mlp/{k}") for k in proposal.MLP_KEYS})
+        state._load_proposal(fields["params"])
+        for i, (m, v) in enumerate(zip(fields["exp_avg"].leaves(), fields["exp_avg_sq"].leaves())):
+            state.prop_flat.view(state.prop_flat.exp_avg, i).copy_(m)
+            state.prop_flat.view(state.prop_flat.exp_avg_sq, i).copy_(v)
+    return state
 
 
 def upsampled_grid_dim(config: TensorfConfig, upsamp_index: int) -> int:

@@ -172,19 +172,19 @@ def test_proposal_sample(contraction):
     assert np.all(np.diff(got, axis=1) >= 0)
 
 
-def test_proposal_loss_bwd():
-    log("case: proposal loss bwd")
+@pytest.mark.parametrize("n0,n1,nout,r", [(32, 16, 16, 128), (128, 64, 64, 48), (40, 0, 24, 64)])
+def test_proposal_loss_bwd(n0, n1, nout, r):
+    log(f"case: proposal loss bwd rounds {n0} -> {n1} -> {nout}, R={r}")
     from tensorf_amd import proposal
-    cfg = po.ProposalConfig(samples_round0=32, samples_round1=16, samples_out=16,
+    cfg = po.ProposalConfig(samples_round0=n0, samples_round1=n1, samples_out=nout,
                             histogram_pad=0.02, loss_weight=0.7)
     pp = _prop_params(seed=6)
-    r = 128
     o, d = _rays(r, 6)
-    jit = f32(np.random.default_rng(6).uniform(size=(r, 48)))
+    jit = f32(np.random.default_rng(6).uniform(size=(r, n0 + n1)))
     s, field, box, keep = _sample(pp, cfg, o, d, jit)
     fe = s.edges
     rng = np.random.default_rng(7)
-    fw = f32(rng.uniform(0, 0.3, (r, 16)) * (rng.uniform(size=(r, 16)) < 0.5))
+    fw = f32(rng.uniform(0, 0.3, (r, nout)) * (rng.uniform(size=(r, nout)) < 0.5) / (nout / 16))
     grads = proposal.ProposalField(
         type(field.tensor)(type(field.tensor.stacked_single_vm)(
             torch.zeros_like(field.tensor.stacked_single_vm.plane),
@@ -193,7 +193,7 @@ def test_proposal_loss_bwd():
     fw_d = dev(fw)
     loss = s.loss_backward(fe, fw_d, grads)
     torch.cuda.synchronize()
-    gpu_edges = [s.round(k)[0].cpu().numpy().astype(np.float64) for k in range(2)]
+    gpu_edges = [s.round(k)[0].cpu().numpy().astype(np.float64) for k in range(1 + (n1 > 0))]
     fe_np = fe.cpu().numpy().astype(np.float64)
     _, rounds = po.proposal_rounds(pp, cfg, box, o, d, jit, None, None, gpu_edges)
     lref, gref = po.proposal_loss_backward(pp, rounds, cfg, fe_np, fw)
