@@ -276,7 +276,7 @@ int32_t tfh_train_step(const tfh_vm* density, const tfh_vm* appearance, const tf
  * composites their weights w_j over its intervals (the reference's
  * compute_segment_probabilities, render.py:300-347) and places the next
  * round's interval edges at the quantiles m/n' of the piecewise-linear CDF
- * of (w_j + pad / n_j) (the edges of the round's span are kept). The last
+ * of (w_j^power + pad / n_j) (the edges of the round's span are kept). The last
  * round's edges (R, samples_out + 1) are the field's intervals
  * (TFH_SAMPLING_EDGES). Training (mip-NeRF 360's interval loss) takes the
  * field's weights w_i on its intervals as constants and, for every round,
@@ -304,6 +304,8 @@ typedef struct {
   int32_t scene_contraction; /* as tfh_render_config */
   float histogram_pad;      /* pad mass added to each round's weights before resampling */
   float loss_weight;        /* weight of the interval loss */
+  float weight_power;       /* resample on w^power (0: evenly, 1: the weights; a
+                               training schedule ramps it from 0 to 1) */
 } tfh_proposal_config;
 
 /* The proposal MLP alone: features (M, feat_dim) -> z (M) (may be NULL) and
@@ -320,11 +322,11 @@ int32_t tfh_prop_mlp_bwd(const tfh_prop_mlp* mlp, const float* features, int64_t
 
 /* Inverse-CDF resampling of one round: for each ray, edges (n+1) and
  * weights (n) -> edges_out (num_out+1) at the quantiles m/num_out of the
- * piecewise-linear CDF of (w_j + pad/n) over the edges; edges_out[0] and
- * edges_out[num_out] repeat the ray's first and last edge. */
+ * piecewise-linear CDF of (w_j^power + pad/n) over the edges; edges_out[0]
+ * and edges_out[num_out] repeat the ray's first and last edge. */
 int32_t tfh_proposal_resample(const float* edges, const float* weights, int64_t num_rays,
-                              int64_t num_intervals, int64_t num_out, float pad, float* edges_out,
-                              tfh_stream_t stream);
+                              int64_t num_intervals, int64_t num_out, float pad, float power,
+                              float* edges_out, tfh_stream_t stream);
 /* Interval loss per ray (see above) of field intervals edges (R, n+1) with
  * weights (R, n) against proposal intervals prop_edges (R, n_p+1) with
  * prop_weights (R, n_p), times scale: loss (R) and, when grad_prop_weights

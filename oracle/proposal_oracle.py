@@ -14,7 +14,7 @@ the build's own definition, restated here op for op as the GPU kernels run it:
   n0 equal intervals, or takes the contracted schedule (render.py:127-155) as
   interval starts; each round's weights are compute_segment_probabilities
   (render.py:300-347) over its intervals; the next round's edges are the
-  quantiles m/n' of the piecewise-linear CDF of (w + pad/n);
+  quantiles m/n' of the piecewise-linear CDF of (w^power + pad/n);
 * mip-NeRF 360's interval loss (Barron et al. 2022, eq. 13) of every round
   against the field's weights, differentiated w.r.t. the proposal only.
 
@@ -48,6 +48,7 @@ class ProposalConfig:
     scene_contraction: bool = False
     histogram_pad: float = 0.01
     loss_weight: float = 1.0
+    weight_power: float = 1.0
 
 
 def init_prop_params(grid_dim, channels, rng: np.random.Generator, std=0.3):
@@ -122,14 +123,14 @@ def round_points(origins, directions, edges, jitter, aabb, contraction, dtype=np
     return T.normalize_points(pts, aabb, dtype), steps
 
 
-def resample(edges, w, n_out, pad, dtype=np.float64):
-    """Inverse-CDF edges (R, n_out+1): h = w + pad/n, F = [0, cumsum(h)], out[m] =
+def resample(edges, w, n_out, pad, dtype=np.float64, power=1.0):
+    """Inverse-CDF edges (R, n_out+1): h = w^power + pad/n, F = [0, cumsum(h)], out[m] =
     F^{-1}(F_n m / n_out) piecewise linearly between the edges, out[0] = e_0,
     out[n_out] = e_n."""
     e = T._c(edges, dtype)
     w = T._c(w, dtype)
     r, n = w.shape
-    h = w + dtype(pad) / dtype(n)
+    h = (w if power == 1.0 else np.power(w, power)) + dtype(pad) / dtype(n)
     cdf = np.concatenate([np.zeros((r, 1), dtype), np.cumsum(h, axis=1)], axis=1)
     out = np.empty((r, n_out + 1), dtype)
     out[:, 0] = e[:, 0]
@@ -214,7 +215,7 @@ def proposal_rounds(params, cfg: ProposalConfig, aabb, origins, directions, jitt
         rounds.append(dict(edges=edges, xhat=xhat, steps=steps, feats=feats, z=z, sigma=sigma,
                            w=w, e=e, mlp=cache))
         n_next = ns[k + 1] if k + 1 < len(ns) else cfg.samples_out
-        edges = resample(edges, w, n_next, cfg.histogram_pad, dtype)
+        edges = resample(edges, w, n_next, cfg.histogram_pad, dtype, cfg.weight_power)
         joff += n
     return edges, rounds
 

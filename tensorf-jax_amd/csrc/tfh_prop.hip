@@ -32,15 +32,18 @@ int32_t prop_mlp_bwd(const tfh_prop_mlp* p, const float* feats, const float* dz,
 constexpr float kLossEps = 1e-7f;
 
 // ---- device building blocks (one wave per ray) ---------------------------------------
-// Inverse-CDF edges: h_j = w_j + pad / n, F = prefix sums of h, out[m] =
+// Inverse-CDF edges: h_j = w_j^power + pad / n, F = prefix sums of h, out[m] =
 // F^{-1}(m / n_out * F_n) on the piecewise-linear CDF over the edges, with
 // out[0] = edge[0] and out[n_out] = edge[n].
 __device__ inline void wave_resample(const float* s_edge, const float* s_w, int n, int n_out,
-                                     float pad, float* s_h, float* s_cdf,
+                                     float pad, float power, float* s_h, float* s_cdf,
                                      float* __restrict__ out) {
   const int lane = lane_id();
   const float floor_mass = pad / (float)n;
-  for (int j = lane; j < n; j += 64) s_h[j] = s_w[j] + floor_mass;
+  for (int j = lane; j < n; j += 64) {
+    const float w = s_w[j];
+    s_h[j] = (power == 1.0f ? w : (power == 0.0f ? 1.0f : powf(w, power))) + floor_mass;
+  }
   __syncthreads();
   wave_excl_prefix(s_h, s_cdf, n);
   __syncthreads();
@@ -68,23 +71,24 @@ __device__ inline void wave_resample(const float* s_edge, const float* s_w, int 
 // (i < nf), proposal intervals [pe_j, pe_{j+1}) with weights pw_j (j < n):
 //   bound_i = sum of pw_j over the j with pe_{j+1} > fe_i and pe_j < fe_{i+1}
 //   loss = scale * sum_i max(0, fw_i - bound_i)^2 / (fw_i + eps)
-// and s_G[j] = dloss/dpw_j. Returns the loss (wave-uniform). Scratch: s_W
-// (n+1), s_g (nf). The field intervals overlapping proposal interval j are
-// the index range [c_hi, c_lo) (both edge arrays are sorted); s_G[j] sums
-// their loss gradients directly (a difference of prefix sums would cancel
-// badly in FP32). Over all j the ranges hold about nf + n entries.
+// and s_G[j] = dloss/dpw_j. Returns the loss (wave-uniform). Scratch: s_g
+// (nf). Both edge arrays are sorted, so the proposal intervals overlapping
+// field interval i are an index range [jlo, jend) and the field intervals
+// overlapping proposal interval j are a range [c_hi, c_lo); both sums run
+// over their ranges directly (about nf + n terms in all). Differences of
+// prefix sums would cancel in FP32, and dloss/dbound = -2 (w - bound) / w
+// amplifies an absolute error of the bound by 1/w.
 __device__ inline float wave_interval_loss(const float* s_fe, const float* __restrict__ fw, int nf,
                                            const float* s_pe, const float* s_pw, int n,
-                                           float scale, float* s_W, float* s_g, float* s_G) {
+                                           float scale, float* s_g, float* s_G) {
   const int lane = lane_id();
-  wave_excl_prefix(s_pw, s_W, n);
-  __syncthreads();
   float loss = 0.0f;
   for (int i = lane; i < nf; i += 64) {
     const float a = s_fe[i], b = s_fe[i + 1], wi = fw[i];
     const int jlo = upper_bound(s_pe + 1, n, a);   // first j with pe_{j+1} > a
     const int jend = lower_bound(s_pe, n, b);      // one past the last j with pe_j < b
-    const float bound = jend > jlo ? s_W[jend] - s_W[jlo] : 0.0f;
+    float bound = 0.0f;
+    for (int j = jlo; j < jend; ++j) bound += s_pw[j];
     const float dlt = fmaxf(wi - bound, 0.0f);
     const float inv = 1.0f / (wi + kLossEps);
     loss += dlt * dlt * inv;
@@ -155,7 +159,7 @@ __global__ void __launch_bounds__(256) k_prop_points(PropPts p) {
 // Weights of the round from sigma and its edges, then the next round's edges.
 __global__ void __launch_bounds__(64) k_prop_resample(const float* __restrict__ edges,
                                                       const float* __restrict__ sigma, int n,
-                                                      int n_out, float pad,
+                                                      int n_out, float pad, float power,
                                                       float* __restrict__ w_out,
                                                       float* __restrict__ e_out,
                                                       float* __restrict__ next_edges) {
@@ -177,18 +181,17 @@ __global__ void __launch_bounds__(64) k_prop_resample(const float* __restrict__ 
     w_out[r * n + j] = s_w[j];
     e_out[r * n + j] = s_e[j];
   }
-  wave_resample(s_edge, s_w, n, n_out, pad, s_a, s_cdf, next_edges + r * (n_out + 1));
+  wave_resample(s_edge, s_w, n, n_out, pad, power, s_a, s_cdf, next_edges + r * (n_out + 1));
 }
 
 // LDS layout (in floats) of k_prop_loss_bwd, shared by the kernel and its launch.
 struct LossLds {
-  int fe, pe, pw, W, g, G, e, da, total;
+  int fe, pe, pw, g, G, e, da, total;
   __host__ __device__ LossLds(int nf, int n) {
     fe = 0;             // nf+1 field edges
     pe = fe + nf + 1;   // n+1 round edges
     pw = pe + n + 1;    // n round weights
-    W = pw + n;         // n+1 prefix sums of the round weights
-    g = W + n + 1;      // nf loss gradients of the field intervals
+    g = pw + n;         // nf loss gradients of the field intervals
     G = g + nf;         // n dloss/dw of the round
     e = G + n;          // n round transmittances
     da = e + n;         // n dloss/da of the round
@@ -208,7 +211,6 @@ __global__ void __launch_bounds__(64) k_prop_loss_bwd(
   float* s_fe = sm + L.fe;
   float* s_pe = sm + L.pe;
   float* s_pw = sm + L.pw;
-  float* s_W = sm + L.W;
   float* s_g = sm + L.g;
   float* s_G = sm + L.G;
   float* s_e = sm + L.e;
@@ -221,7 +223,7 @@ __global__ void __launch_bounds__(64) k_prop_loss_bwd(
   }
   __syncthreads();
   const float loss =
-      wave_interval_loss(s_fe, fw + r * nf, nf, s_pe, s_pw, n, scale, s_W, s_g, s_G);
+      wave_interval_loss(s_fe, fw + r * nf, nf, s_pe, s_pw, n, scale, s_g, s_G);
   if (threadIdx.x == 0 && loss != 0.0f) unsafeAtomicAdd(loss_out, loss);
   wave_segment_vjp(s_G, nullptr, s_pw, s_e, s_da, n);
   __syncthreads();
@@ -235,7 +237,7 @@ __global__ void __launch_bounds__(64) k_prop_loss_bwd(
 // ---- standalone kernels -------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_resample_api(const float* __restrict__ edges,
                                                      const float* __restrict__ w, int n,
-                                                     int n_out, float pad,
+                                                     int n_out, float pad, float power,
                                                      float* __restrict__ out) {
   extern __shared__ float sm[];
   const int64_t r = blockIdx.x;
@@ -246,7 +248,7 @@ __global__ void __launch_bounds__(64) k_resample_api(const float* __restrict__ e
   for (int j = threadIdx.x; j <= n; j += 64) s_edge[j] = edges[r * (n + 1) + j];
   for (int j = threadIdx.x; j < n; j += 64) s_w[j] = w[r * n + j];
   __syncthreads();
-  wave_resample(s_edge, s_w, n, n_out, pad, s_h, s_cdf, out + r * (n_out + 1));
+  wave_resample(s_edge, s_w, n, n_out, pad, power, s_h, s_cdf, out + r * (n_out + 1));
 }
 
 __global__ void __launch_bounds__(64) k_interval_loss_api(
@@ -258,15 +260,14 @@ __global__ void __launch_bounds__(64) k_interval_loss_api(
   float* s_fe = sm;
   float* s_pe = s_fe + nf + 1;
   float* s_pw = s_pe + n + 1;
-  float* s_W = s_pw + n;
-  float* s_g = s_W + n + 1;
+  float* s_g = s_pw + n;
   float* s_G = s_g + nf;
   for (int i = threadIdx.x; i <= nf; i += 64) s_fe[i] = fedges[r * (nf + 1) + i];
   for (int j = threadIdx.x; j <= n; j += 64) s_pe[j] = pedges[r * (n + 1) + j];
   for (int j = threadIdx.x; j < n; j += 64) s_pw[j] = pw[r * n + j];
   __syncthreads();
   const float l =
-      wave_interval_loss(s_fe, fw + r * nf, nf, s_pe, s_pw, n, scale, s_W, s_g, s_G);
+      wave_interval_loss(s_fe, fw + r * nf, nf, s_pe, s_pw, n, scale, s_g, s_G);
   if (threadIdx.x == 0) loss[r] = l;
   if (gpw)
     for (int j = threadIdx.x; j < n; j += 64) gpw[r * n + j] = s_G[j];
@@ -296,6 +297,8 @@ static int32_t check_prop_cfg(const tfh_proposal_config* pc, const char* who) {
   TFH_REQUIRE(pc->samples_out >= 1 && pc->samples_out <= kMaxSamples,
               "%s: samples_out %d outside [1, %d]", who, pc->samples_out, kMaxSamples);
   TFH_REQUIRE(pc->histogram_pad >= 0.0f, "%s: negative histogram_pad", who);
+  TFH_REQUIRE(pc->weight_power >= 0.0f && pc->weight_power <= 1.0f,
+              "%s: weight_power %g outside [0, 1]", who, pc->weight_power);
   return TFH_OK;
 }
 
@@ -371,7 +374,8 @@ int32_t prop_sample(const tfh_vm* field, const tfh_prop_mlp* mlp, const tfh_prop
       ProfScope ps("prop_resample", st);
       const size_t lds = (size_t)(4 * rd.n + 2) * sizeof(float);
       hipLaunchKernelGGL(k_prop_resample, dim3((unsigned)R), dim3(64), lds, st, rd.edges,
-                         rd.sigma, rd.n, n_out, pc->histogram_pad, rd.w, rd.e, next);
+                         rd.sigma, rd.n, n_out, pc->histogram_pad, pc->weight_power, rd.w, rd.e,
+                         next);
     }
     TFH_TRY(check_launch("proposal resample"));
     joff += rd.n;
@@ -494,18 +498,19 @@ int32_t tfh_proposal_loss_bwd(const tfh_vm* field, const tfh_prop_mlp* mlp,
 }
 
 int32_t tfh_proposal_resample(const float* edges, const float* weights, int64_t num_rays,
-                              int64_t num_intervals, int64_t num_out, float pad, float* edges_out,
-                              tfh_stream_t stream) {
+                              int64_t num_intervals, int64_t num_out, float pad, float power,
+                              float* edges_out, tfh_stream_t stream) {
   TFH_REQUIRE(num_rays >= 0 && num_intervals >= 1 && num_intervals <= 4096 && num_out >= 1 &&
                   num_out <= 4096,
               "tfh_proposal_resample: bad sizes");
   TFH_REQUIRE(pad >= 0.0f, "tfh_proposal_resample: negative pad");
+  TFH_REQUIRE(power >= 0.0f && power <= 1.0f, "tfh_proposal_resample: power outside [0, 1]");
   if (num_rays == 0) return TFH_OK;
   TFH_REQUIRE(edges && weights && edges_out, "tfh_proposal_resample: null pointer");
   TFH_TRY(require_device());
   const size_t lds = (size_t)(4 * num_intervals + 2) * sizeof(float);
   hipLaunchKernelGGL(k_resample_api, dim3((unsigned)num_rays), dim3(64), lds, as_stream(stream),
-                     edges, weights, (int)num_intervals, (int)num_out, pad, edges_out);
+                     edges, weights, (int)num_intervals, (int)num_out, pad, power, edges_out);
   return check_launch("tfh_proposal_resample");
 }
 

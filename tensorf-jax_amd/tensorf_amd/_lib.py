@@ -73,7 +73,8 @@ class PropMlpGrad(ctypes.Structure):
 
 class ProposalConfig(ctypes.Structure):
     _fields_ = [("samples_round0", c_i32), ("samples_round1", c_i32), ("samples_out", c_i32),
-                ("scene_contraction", c_i32), ("histogram_pad", c_f), ("loss_weight", c_f)]
+                ("scene_contraction", c_i32), ("histogram_pad", c_f), ("loss_weight", c_f),
+                ("weight_power", c_f)]
 
 
 class TrainBuffers(ctypes.Structure):
@@ -117,7 +118,7 @@ _SIGS = {
     "tfh_prop_mlp_bwd_workspace_size": (c_sz, [P(PropMlp), c_i64]),
     "tfh_prop_mlp_bwd": (c_i32, [P(PropMlp), c_p, c_i64, c_p, c_p, P(PropMlpGrad), c_p, c_sz,
                                  c_p]),
-    "tfh_proposal_resample": (c_i32, [c_p, c_p, c_i64, c_i64, c_i64, c_f, c_p, c_p]),
+    "tfh_proposal_resample": (c_i32, [c_p, c_p, c_i64, c_i64, c_i64, c_f, c_f, c_p, c_p]),
     "tfh_interval_loss": (c_i32, [c_p, c_p, c_i64, c_p, c_p, c_i64, c_i64, c_f, c_p, c_p, c_p]),
     "tfh_proposal_workspace_size": (c_sz, [P(ProposalConfig), P(Vm), c_i64]),
     "tfh_proposal_sample": (c_i32, [P(Vm), P(PropMlp), P(ProposalConfig), c_p, c_p, c_p, c_i64,

@@ -36,11 +36,21 @@ class ProposalConfig:
     grid_dim: int = 64
     channels: int = 8
     init_std: float = 0.3
+    # resampling uses w^power; training ramps power linearly from 0 (even
+    # resampling) to 1 over the first anneal_steps steps so that the field sees
+    # samples along the whole ray while the proposal is still untrained
+    anneal_steps: int = 1000
 
-    def c_struct(self, contraction: bool) -> _lib.ProposalConfig:
+    def weight_power(self, step: Optional[int]) -> float:
+        if step is None or self.anneal_steps <= 0:
+            return 1.0
+        return min(1.0, step / self.anneal_steps)
+
+    def c_struct(self, contraction: bool, power: float = 1.0) -> _lib.ProposalConfig:
         return _lib.ProposalConfig(int(self.samples_round0), int(self.samples_round1),
                                    int(self.samples_out), int(bool(contraction)),
-                                   float(self.histogram_pad), float(self.loss_weight))
+                                   float(self.histogram_pad), float(self.loss_weight),
+                                   float(power))
 
     def jitter_count(self) -> int:
         return self.samples_round0 + self.samples_round1
@@ -153,15 +163,15 @@ class Sampler:
         return loss
 
 
-def resample(edges: torch.Tensor, weights: torch.Tensor, num_out: int, pad: float
-             ) -> torch.Tensor:
+def resample(edges: torch.Tensor, weights: torch.Tensor, num_out: int, pad: float,
+             power: float = 1.0) -> torch.Tensor:
     """Inverse-CDF resampling of per-ray histograms: (R, n+1), (R, n) -> (R, num_out+1)."""
     r, n = weights.shape
     e = edges.to(torch.float32).contiguous()
     w = weights.to(torch.float32).contiguous()
     out = torch.empty((r, num_out + 1), dtype=torch.float32, device=e.device)
     _lib.call("tfh_proposal_resample", _lib.ptr(e), _lib.ptr(w), r, n, num_out, float(pad),
-              _lib.ptr(out), _lib.stream())
+              float(power), _lib.ptr(out), _lib.stream())
     return out
 
 

@@ -285,7 +285,7 @@ class TrainState:
         jitter = render.uniform(seed, r * (pc.jitter_count() + n), env.device)
         field = lp.proposal_field
         pvm, pmlp = field.c_vm(), field.c_mlp()
-        cpc = pc.c_struct(env.contraction)
+        cpc = pc.c_struct(env.contraction, pc.weight_power(self.step))
         ca = lp.appearance_tensor.stacked_single_vm.channel_dim()
         need = int(_lib.lib.tfh_train_step_proposal_workspace_size(
             ctypes.byref(env.cfg), ctypes.byref(env.cmlp), ca, ctypes.byref(cpc),

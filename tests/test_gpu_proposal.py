@@ -61,17 +61,18 @@ def test_prop_mlp_fwd_bwd():
     assert e.max() <= 1e-4 * np.abs(gfr).max()
 
 
-@pytest.mark.parametrize("n,n_out,pad", [(37, 23, 0.01), (128, 64, 0.0), (200, 300, 0.05)])
-def test_resample(n, n_out, pad):
-    log(f"case: resample n={n} n_out={n_out} pad={pad}")
+@pytest.mark.parametrize("n,n_out,pad,power", [(37, 23, 0.01, 1.0), (128, 64, 0.0, 1.0),
+                                               (200, 300, 0.05, 0.5), (50, 20, 0.01, 0.0)])
+def test_resample(n, n_out, pad, power):
+    log(f"case: resample n={n} n_out={n_out} pad={pad} power={power}")
     from tensorf_amd import proposal
     rng = np.random.default_rng(n)
     r = 97
     e = _sorted_edges(rng, r, n)
     w = rng.exponential(size=(r, n)) * (rng.uniform(size=(r, n)) < 0.3)
     w = f32(w / (w.sum(1, keepdims=True) * 1.1 + 1e-12))
-    got = proposal.resample(dev(e), dev(w), n_out, pad).cpu().numpy().astype(np.float64)
-    want = po.resample(e, w, n_out, pad)
+    got = proposal.resample(dev(e), dev(w), n_out, pad, power).cpu().numpy().astype(np.float64)
+    want = po.resample(e, w, n_out, pad, power=power)
     err = report("edges", got, want)
     assert err.max() <= 2e-5 * (e[:, -1] - e[:, 0]).max()
     assert np.all(np.diff(got, axis=1) >= 0)
@@ -251,7 +252,7 @@ def test_train_step_proposal():
     shape = orc.MlpShape()
     g, cd, ca, r = 16, 16, 48, 256
     pcfg = proposal.ProposalConfig(samples_round0=32, samples_round1=16, samples_out=24,
-                                   histogram_pad=0.02, grid_dim=12, channels=4)
+                                   histogram_pad=0.02, grid_dim=12, channels=4, anneal_steps=0)
     cfg = training.TensorfConfig(minibatch_size=r, n_iters=100, initial_aabb_min=(-1.5,) * 3,
                                  initial_aabb_max=(1.5,) * 3, appearance_feat_dim=ca,
                                  density_feat_dim=cd, feature_n_freqs=2, viewdir_n_freqs=2,

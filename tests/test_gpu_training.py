@@ -126,7 +126,8 @@ def test_proposal_beats_uniform_at_equal_samples():
     res = {}
     for name, pc in (("uniform", None),
                      ("proposal", proposal.ProposalConfig(samples_round0=64, samples_round1=32,
-                                                          samples_out=32, grid_dim=32))):
+                                                          samples_out=32, grid_dim=32,
+                                                          anneal_steps=100))):
         cfg = _config(g=48, proposal=pc, n=32, steps=300)
         st = training.TrainState.initialize(cfg, 48, prng_key=3, num_cameras=16)
         batches = data.minibatches(1024, seed=1)

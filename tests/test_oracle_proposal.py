@@ -17,15 +17,16 @@ def _sorted_edges(rng, r, n, lo=1.0, hi=5.0):
     return e
 
 
-def test_resample_matches_interp():
+@pytest.mark.parametrize("power", [1.0, 0.5, 0.0])
+def test_resample_matches_interp(power):
     """The inverse of a piecewise-linear CDF is numpy.interp with the axes swapped."""
     rng = np.random.default_rng(0)
     r, n, n_out, pad = 20, 37, 23, 0.01
     e = _sorted_edges(rng, r, n)
     w = rng.exponential(size=(r, n)) * (rng.uniform(size=(r, n)) < 0.3)
     w /= w.sum(1, keepdims=True) * 1.2
-    out = po.resample(e, w, n_out, pad)
-    h = w + pad / n
+    out = po.resample(e, w, n_out, pad, power=power)
+    h = w ** power + pad / n
     for ri in range(r):
         cdf = np.concatenate([[0.0], np.cumsum(h[ri])])
         q = cdf[-1] * np.arange(n_out + 1) / n_out
