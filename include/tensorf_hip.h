@@ -113,6 +113,9 @@ typedef struct {
   int32_t scene_contraction;          /* 0: bounded box, 1: L-inf contraction */
   int32_t mlp_precision;              /* tfh_precision */
   int32_t sampling;                   /* tfh_sampling */
+  float density_shift;                /* sigma = softplus(sum + shift); the reference
+                                         uses +10 (render.py:207), TensoRF's
+                                         original code -10 */
 } tfh_render_config;
 
 /* ---- library ------------------------------------------------------------ */

@@ -492,6 +492,7 @@ class RenderConfig:
     appearance_samples_per_ray: int = 64
     scene_contraction: bool = False
     sampling: int = 0  # 0: the reference's samplers, 1: per-ray interval edges
+    density_shift: float = 10.0  # render.py:207
 
 
 SAMPLING_SCHEDULE, SAMPLING_EDGES = 0, 1
@@ -520,7 +521,7 @@ def render_rays(params, aabb, origins, directions, camera_indices, cfg: RenderCo
     xhat = normalize_points(pts, aabb, dtype)                        # render.py:193-195
     flat = xhat.reshape(3, r * n)
     dfeat = vm_interpolate(params["density_vector"], params["density_matrix"], flat, dtype)
-    s = dfeat.sum(axis=0).reshape(r, n) + dtype(10.0)                # render.py:207
+    s = dfeat.sum(axis=0).reshape(r, n) + dtype(cfg.density_shift)  # render.py:207
     sigma = softplus(s).astype(dtype)
     p_exits, w = compute_segment_probabilities(sigma, steps, dtype)  # render.py:211
     cache = dict(xhat=xhat, ts=ts, steps=steps, s=s, sigma=sigma, p_exits=p_exits, w=w, d=d,

@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(64) k_density_pass(tfh_vm dvm, RayIn in, Densi
     }
     acc = group16_sum(acc);                        // sum over the 3C channels
     if (act && l16 == 0) {
-      const float sig = softplus(acc + 10.0f);     // render.py:207
+      const float sig = softplus(acc + in.density_shift);  // render.py:207 (+10 there)
       s_a[i] = sig * dlt;                          // render.py:322
       const int64_t s = r * N + i;
       o.sigma[s] = sig;
@@ -527,6 +527,7 @@ static RayIn make_rayin(const tfh_render_config* cfg, const float* aabb, const f
   in.mode = cfg->mode;
   in.contraction = cfg->scene_contraction ? 1 : 0;
   in.sampling = cfg->sampling;
+  in.density_shift = cfg->density_shift;
   in.jitter = jitter;
   in.base_ts = base_ts;
   in.base_steps = base_steps;

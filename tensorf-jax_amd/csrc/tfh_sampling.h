@@ -56,6 +56,7 @@ struct RayIn {
   const float* aabb;  // (2,3) = 6 floats
   int64_t R;
   int N, k, mode, contraction, sampling;
+  float density_shift;      // softplus(sum + shift), +10 in the reference
   const float* jitter;      // (N) bbox, (R,N) contraction or edges, or null
   const float* base_ts;     // (N) contraction schedule, or (R, N+1) interval edges
   const float* base_steps;  // (N)

@@ -60,7 +60,8 @@ class MlpGrad(ctypes.Structure):
 class RenderConfig(ctypes.Structure):
     _fields_ = [("near_plane", c_f), ("far_plane", c_f), ("mode", c_i32),
                 ("density_samples_per_ray", c_i32), ("appearance_samples_per_ray", c_i32),
-                ("scene_contraction", c_i32), ("mlp_precision", c_i32), ("sampling", c_i32)]
+                ("scene_contraction", c_i32), ("mlp_precision", c_i32), ("sampling", c_i32),
+                ("density_shift", c_f)]
 
 
 class PropMlp(ctypes.Structure):

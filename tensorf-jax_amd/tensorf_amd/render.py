@@ -37,6 +37,9 @@ class RenderConfig:
     density_samples_per_ray: int
     appearance_samples_per_ray: int
     appearance_sampling: str = "topk"  # "topk" (deterministic) or "gumbel"
+    # sigma = softplus(sum of density features + density_shift): +10 in the
+    # reference (render.py:207); TensoRF's original code starts transparent with -10
+    density_shift: float = 10.0
     # Learned proposal sampler (no reference counterpart, render.py:126): when
     # set, density_samples_per_ray must equal proposal.samples_out and
     # LearnableParams.proposal_field holds its parameters.
@@ -128,7 +131,7 @@ def c_config(config: RenderConfig, contraction: bool, precision: int) -> _lib.Re
     return _lib.RenderConfig(float(config.near), float(config.far), config.mode.value,
                              int(config.density_samples_per_ray),
                              int(config.appearance_samples_per_ray), int(bool(contraction)),
-                             int(precision), sampling)
+                             int(precision), sampling, float(config.density_shift))
 
 
 def proposal_noise(prng_key, ray_count: int, pc: proposal.ProposalConfig, device):

@@ -63,6 +63,9 @@ class TensorfConfig:
     # Learned proposal sampler (no reference counterpart): when set, the field
     # takes proposal.samples_out samples per ray on the proposal's intervals.
     proposal: Optional[proposal.ProposalConfig] = None
+    # sigma = softplus(features + density_shift); +10 is the reference's
+    # (render.py:207)
+    density_shift: float = 10.0
 
 
 def lr_decay_coeff(step: int, config: TensorfConfig) -> float:
@@ -220,7 +223,8 @@ class TrainState:
     def render_config(self, mode=render.RenderMode.RGB) -> render.RenderConfig:
         n, k = sample_counts(self.config, self.flat.shapes[0][1])
         return render.RenderConfig(self.config.render_near, self.config.render_far, mode, n, k,
-                                   proposal=self.config.proposal)
+                                   proposal=self.config.proposal,
+                                   density_shift=self.config.density_shift)
 
     # ---- training ------------------------------------------------------------------
     def training_step(self, minibatch: cameras.RenderedRays) -> Tuple["TrainState", Dict]:

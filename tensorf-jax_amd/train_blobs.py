@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--grid-final", type=int, default=300)
     ap.add_argument("--upsamp", default="2000,3000,4000,5500,7000")
     ap.add_argument("--box", type=float, default=2.0, help="half extent of the scene box")
+    ap.add_argument("--density-shift", type=float, default=10.0,
+                    help="sigma = softplus(features + shift); +10 is the reference's")
     ap.add_argument("--gt-samples", type=int, default=4096)
     ap.add_argument("--log-every", type=int, default=500)
     ap.add_argument("--chunk", type=int, default=65536)
@@ -88,7 +90,7 @@ def main():
         appearance_feat_dim=48, density_feat_dim=16, feature_n_freqs=2, viewdir_n_freqs=2,
         grid_dim_init=args.grid_init, grid_dim_final=args.grid_final, upsamp_iters=upsamp,
         density_samples_per_ray=None if pc else n, appearance_samples_per_ray=k,
-        mlp_precision=args.precision, proposal=pc)
+        mlp_precision=args.precision, proposal=pc, density_shift=args.density_shift)
 
     t_train = [0.0]
     last = [time.perf_counter()]
@@ -138,8 +140,19 @@ def main():
         depth_miss.append(float((fg & ~np.isfinite(dep)).sum() / max(fg.sum(), 1)))
         log(f"test view {i}: psnr={psnrs[-1]:.3f} median-depth |err|={depth_err[-1]:.4f} "
             f"missed={depth_miss[-1]:.4f}")
+    # one training view, to show the gap between fitted and held-out views
+    rays = cameras.pinhole_rays(train_c2ws[0], args.test_res, args.test_res,
+                                focal * args.test_res / args.res, 0)
+    gt, _ = scenes.render_ground_truth(scene, rays, args.gt_samples)
+    pred = torch.tensor(render.render_rays_batched(state.appearance_mlp, state.learnable_params,
+                                                   state.aabb, rays, 0, rgb_cfg,
+                                                   batch_size=args.chunk), device=gt.device)
+    train_view_psnr = utils.psnr_from_mse(float(((pred - gt) ** 2).mean()))
+    log(f"training view 0: psnr={train_view_psnr:.3f}")
     res = {
         "sampler": args.sampler, "precision": args.precision, "rays_per_step": args.rays,
+        "train_view_psnr": round(train_view_psnr, 4),
+        "density_shift": args.density_shift,
         "samples_per_ray": n, "appearance_samples_per_ray": k,
         "proposal_rounds": args.prop_rounds if pc else None, "steps": args.steps,
         "train_views": len(train_c2ws), "res": args.res, "grid": [args.grid_init, args.grid_final],

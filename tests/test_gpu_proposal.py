@@ -74,7 +74,9 @@ def test_resample(n, n_out, pad, power):
     got = proposal.resample(dev(e), dev(w), n_out, pad, power).cpu().numpy().astype(np.float64)
     want = po.resample(e, w, n_out, pad, power=power)
     err = report("edges", got, want)
-    assert err.max() <= 2e-5 * (e[:, -1] - e[:, 0]).max()
+    # the FP32 CDF carries ~n ulps of its total; an edge moves by that error
+    # over the local histogram height (>= pad / n) times the interval width
+    assert err.max() <= 1e-4 * (e[:, -1] - e[:, 0]).max()
     assert np.all(np.diff(got, axis=1) >= 0)
     np.testing.assert_array_equal(got[:, 0], e[:, 0])
     np.testing.assert_array_equal(got[:, -1], e[:, -1])
