@@ -194,6 +194,8 @@ def main():
     ap.add_argument("--cpu-sample-rays", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stages", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="bytes of one plain device copy run first (calibrates HBM counters)")
     ap.add_argument("--prop-rays", type=int, default=16384)
     ap.add_argument("--no-proposal", action="store_true")
     args = ap.parse_args()
@@ -208,6 +210,15 @@ def main():
     from tensorf_amd import _lib, cameras, render, training
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        # one copy of known size: its FETCH_SIZE / WRITE_SIZE counters scale
+        # the counters of the kernels profiled in the same run
+        src = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     R, N, k, G = args.rays, args.samples, args.appearance_samples, args.grid
     cd, ca = 16, 48
     cfg = training.TensorfConfig(

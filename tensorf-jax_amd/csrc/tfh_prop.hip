@@ -156,6 +156,19 @@ __global__ void __launch_bounds__(256) k_prop_points(PropPts p) {
   }
 }
 
+// LDS layout (in floats) of k_prop_resample, shared by the kernel and its launch.
+struct ResampleLds {
+  int edge, a, e, w, cdf, total;
+  __host__ __device__ explicit ResampleLds(int n) {
+    edge = 0;          // n+1 interval edges
+    a = edge + n + 1;  // n optical depths, then the histogram
+    e = a + n;         // n transmittances
+    w = e + n;         // n weights
+    cdf = w + n;       // n+1 histogram prefix sums
+    total = cdf + n + 1;
+  }
+};
+
 // Weights of the round from sigma and its edges, then the next round's edges.
 __global__ void __launch_bounds__(64) k_prop_resample(const float* __restrict__ edges,
                                                       const float* __restrict__ sigma, int n,
@@ -165,11 +178,12 @@ __global__ void __launch_bounds__(64) k_prop_resample(const float* __restrict__ 
                                                       float* __restrict__ next_edges) {
   extern __shared__ float sm[];
   const int64_t r = blockIdx.x;
-  float* s_edge = sm;          // n+1
-  float* s_a = s_edge + n + 1; // n (later: h)
-  float* s_e = s_a + n;        // n
-  float* s_w = s_e + n;        // n
-  float* s_cdf = s_w + n;      // n+1
+  const ResampleLds L(n);
+  float* s_edge = sm + L.edge;
+  float* s_a = sm + L.a;
+  float* s_e = sm + L.e;
+  float* s_w = sm + L.w;
+  float* s_cdf = sm + L.cdf;
   for (int j = threadIdx.x; j <= n; j += 64) s_edge[j] = edges[r * (n + 1) + j];
   __syncthreads();
   for (int j = threadIdx.x; j < n; j += 64)
@@ -372,7 +386,7 @@ int32_t prop_sample(const tfh_vm* field, const tfh_prop_mlp* mlp, const tfh_prop
     float* next = last ? edges_out : ws.rd[k + 1].edges;
     {
       ProfScope ps("prop_resample", st);
-      const size_t lds = (size_t)(4 * rd.n + 2) * sizeof(float);
+      const size_t lds = (size_t)ResampleLds(rd.n).total * sizeof(float);
       hipLaunchKernelGGL(k_prop_resample, dim3((unsigned)R), dim3(64), lds, st, rd.edges,
                          rd.sigma, rd.n, n_out, pc->histogram_pad, pc->weight_power, rd.w, rd.e,
                          next);

@@ -36,6 +36,12 @@ class ProposalConfig:
     grid_dim: int = 64
     channels: int = 8
     init_std: float = 0.3
+    # initial output bias: softplus(-3) ~ 0.05 per unit length keeps the
+    # untrained proposal nearly transparent, so its first resamplings spread
+    # the field's samples along the whole ray (an opaque start concentrates
+    # them at the ray entry, where an empty field never produces the weight
+    # the interval loss needs to move them)
+    init_bias: float = -3.0
     # resampling uses w^power; training ramps power linearly from 0 (even
     # resampling) to 1 over the first anneal_steps steps so that the field sees
     # samples along the whole ray while the proposal is still untrained
@@ -86,7 +92,9 @@ class ProposalField:
         vm = tensor_vm.TensorVM.initialize(config.grid_dim, config.channels,
                                            init=tensor_vm.normal_init(config.init_std),
                                            prng_key=gen, device=device)
-        return ProposalField(vm, init_mlp(3 * config.channels, gen, device))
+        mlp = init_mlp(3 * config.channels, gen, device)
+        mlp["b3"].fill_(config.init_bias)
+        return ProposalField(vm, mlp)
 
     def leaves(self) -> List[torch.Tensor]:
         s = self.tensor.stacked_single_vm
@@ -112,10 +120,10 @@ class Sampler:
 
     def __init__(self, field: ProposalField, config: ProposalConfig, aabb, origins, directions,
                  jitter=None, base_ts=None, base_steps=None, contraction: bool = False,
-                 workspace: Optional[torch.Tensor] = None):
+                 workspace: Optional[torch.Tensor] = None, power: float = 1.0):
         self.field, self.config = field, config
         self.r = origins.shape[0]
-        self.pc = config.c_struct(contraction)
+        self.pc = config.c_struct(contraction, power)
         self.vm = field.c_vm()
         self.cmlp = field.c_mlp()
         self.keep = (aabb, origins, directions, jitter, base_ts, base_steps)

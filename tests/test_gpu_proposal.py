@@ -127,7 +127,7 @@ def _rays(r, seed):
     return f32(o), f32(d)
 
 
-def _sample(pp, cfg, o, d, jit, contraction=False, base_ts=None, base_steps=None):
+def _sample(pp, cfg, o, d, jit, contraction=False, base_ts=None, base_steps=None, power=1.0):
     from tensorf_amd import proposal
     field = _device_field(pp)
     box = BOX if not contraction else np.array([[-2.0] * 3, [2.0] * 3])
@@ -137,24 +137,28 @@ def _sample(pp, cfg, o, d, jit, contraction=False, base_ts=None, base_steps=None
     pcfg = proposal.ProposalConfig(cfg.samples_round0, cfg.samples_round1, cfg.samples_out,
                                    cfg.histogram_pad, cfg.loss_weight,
                                    pp["prop_vector"].shape[2], pp["prop_vector"].shape[1])
-    s = proposal.Sampler(field, pcfg, *keep, contraction=contraction)
+    s = proposal.Sampler(field, pcfg, *keep, contraction=contraction, power=power)
     torch.cuda.synchronize()
     return s, field, box, keep
 
 
-@pytest.mark.parametrize("contraction", [False, True])
-def test_proposal_sample(contraction):
-    log(f"case: proposal sample contraction={contraction}")
-    cfg = po.ProposalConfig(samples_round0=32, samples_round1=16, samples_out=16,
-                            scene_contraction=contraction, histogram_pad=0.02)
+@pytest.mark.parametrize("contraction,n0,n1,nout,power", [
+    (False, 32, 16, 16, 1.0), (True, 32, 16, 16, 1.0), (False, 128, 64, 64, 1.0),
+    (False, 128, 64, 64, 0.3), (True, 256, 96, 48, 1.0)])
+def test_proposal_sample(contraction, n0, n1, nout, power):
+    log(f"case: proposal sample contraction={contraction} rounds {n0} -> {n1} -> {nout} "
+        f"power={power}")
+    cfg = po.ProposalConfig(samples_round0=n0, samples_round1=n1, samples_out=nout,
+                            scene_contraction=contraction, histogram_pad=0.02,
+                            weight_power=power)
     pp = _prop_params()
     r = 96
     o, d = _rays(r, 5)
-    jit = f32(np.random.default_rng(5).uniform(size=(r, 48)))
+    jit = f32(np.random.default_rng(5).uniform(size=(r, n0 + n1)))
     bt = bs = None
     if contraction:
-        bt, bs = orc.contraction_schedule(32, 0.05, 200.0)
-    s, field, box, keep = _sample(pp, cfg, o, d, jit, contraction, bt, bs)
+        bt, bs = orc.contraction_schedule(n0, 0.05, 200.0)
+    s, field, box, keep = _sample(pp, cfg, o, d, jit, contraction, bt, bs, power)
     gpu_rounds = [s.round(k) for k in range(2)]
     gpu_edges = [gpu_rounds[k][0].cpu().numpy().astype(np.float64) for k in range(2)]
     # weights at the GPU's own intervals
@@ -304,3 +308,61 @@ def test_train_step_proposal():
         frac_bad = np.mean(e > 1e-3 * lr)
         log(f"  fraction of {key} updates off by >1e-3 lr: {frac_bad:.2e}")
         assert frac_bad < 2e-2
+
+
+def _params_from_state(st):
+    lp = st.learnable_params
+    dv, dm = lp.density_tensor.to_reference_layout()
+    av, am = lp.appearance_tensor.to_reference_layout()
+    mp = lp.appearance_mlp_params["params"]
+    p = {"density_vector": dv, "density_matrix": dm, "appearance_vector": av,
+         "appearance_matrix": am, "w0": mp["Dense_0"]["kernel"], "w1": mp["Dense_1"]["kernel"],
+         "b1": mp["Dense_1"]["bias"], "w2": mp["Dense_2"]["kernel"], "b2": mp["Dense_2"]["bias"],
+         "w3": mp["Dense_3"]["kernel"], "b3": mp["Dense_3"]["bias"]}
+    p = {k: v.detach().cpu().numpy().astype(np.float64) for k, v in p.items()}
+    pv, pm = lp.proposal_field.tensor.to_reference_layout()
+    pp = {"prop_vector": pv, "prop_matrix": pm}
+    pp.update({"p" + k: v for k, v in lp.proposal_field.mlp.items()})
+    pp = {k: v.detach().cpu().numpy().astype(np.float64) for k, v in pp.items()}
+    return p, pp
+
+
+def test_train_steps_proposal_follow_oracle():
+    """Five fused proposal training steps with the weight-power ramp and a
+    transparent start (density shift -10) track the oracle's trajectory."""
+    log("case: 5 proposal train steps vs oracle")
+    from tensorf_amd import cameras, proposal, render, training
+    shape = orc.MlpShape()
+    g, r, steps = 16, 256, 5
+    pcfg = proposal.ProposalConfig(samples_round0=32, samples_round1=16, samples_out=24,
+                                   histogram_pad=0.02, grid_dim=12, channels=4, anneal_steps=3)
+    cfg = training.TensorfConfig(minibatch_size=r, n_iters=100, initial_aabb_min=(-1.5,) * 3,
+                                 initial_aabb_max=(1.5,) * 3, appearance_feat_dim=48,
+                                 density_feat_dim=16, feature_n_freqs=2, viewdir_n_freqs=2,
+                                 grid_dim_init=g, grid_dim_final=g, upsamp_iters=(),
+                                 appearance_samples_per_ray=24, proposal=pcfg,
+                                 density_shift=-10.0)
+    st = training.TrainState.initialize(cfg, g, prng_key=4, num_cameras=1)
+    p, pp = _params_from_state(st)
+    ost, opst = orc.init_opt_state(p), orc.init_opt_state(pp)
+    ocfg = orc.RenderConfig(0.05, 200.0, orc.RGB, 24, 24, sampling=orc.SAMPLING_EDGES,
+                            density_shift=-10.0)
+    opc = po.ProposalConfig(32, 16, 24, False, 0.02, 1.0)
+    for step in range(steps):
+        o, d = _rays(r, 20 + step)
+        colors = f32(0.5 + 0.4 * d)
+        batch = cameras.RenderedRays(dev(colors), cameras.Rays3D(
+            dev(o), dev(d), torch.zeros(r, dtype=torch.int32, device="cuda")))
+        jit = render.uniform(render._seed(st.prng_key, step), r * (48 + 24)).cpu().numpy()
+        jit = jit.astype(np.float64)
+        st, logs = st.training_step(batch)
+        torch.cuda.synchronize()
+        opc.weight_power = pcfg.weight_power(step)
+        p, pp, ost, opst, ologs, _, _ = po.train_step_proposal(
+            p, pp, ost, opst, step, BOX, o, d, np.zeros(r), colors, ocfg, opc, shape,
+            jit[:r * 48].reshape(r, 48), jit[r * 48:].reshape(r, 24), n_iters=100)
+        mse, pl = float(logs["mse"].item()), float(logs["prop_loss"].item())
+        log(f"  step {step} power {opc.weight_power:.2f}: mse gpu={mse:.8f} "
+            f"oracle={ologs['mse']:.8f}; prop loss gpu={pl:.6e} oracle={ologs['prop_loss']:.6e}")
+        assert abs(mse - ologs["mse"]) <= 1e-3 * ologs["mse"]
+        assert abs(pl - ologs["prop_loss"]) <= 5e-2 * ologs["prop_loss"] + 1e-7
