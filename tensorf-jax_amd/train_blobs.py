@@ -37,6 +37,9 @@ def main():
     ap.add_argument("--appearance-samples", type=int, default=0, help="0 = all samples")
     ap.add_argument("--prop-rounds", default="128,64", help="proposal samples per round")
     ap.add_argument("--prop-out", type=int, default=64, help="field samples with the proposal")
+    ap.add_argument("--prop-pad", type=float, default=0.01, help="histogram pad mass")
+    ap.add_argument("--prop-anneal", type=int, default=1000, help="weight-power ramp steps")
+    ap.add_argument("--prop-grid", type=int, default=64)
     ap.add_argument("--rays", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--views", type=int, default=100)
@@ -81,7 +84,8 @@ def main():
         rounds = [int(x) for x in args.prop_rounds.split(",")]
         pc = proposal.ProposalConfig(samples_round0=rounds[0],
                                      samples_round1=rounds[1] if len(rounds) > 1 else 0,
-                                     samples_out=args.prop_out)
+                                     samples_out=args.prop_out, histogram_pad=args.prop_pad,
+                                     anneal_steps=args.prop_anneal, grid_dim=args.prop_grid)
     n = args.prop_out if pc else args.samples
     k = args.appearance_samples or n
     cfg = training.TensorfConfig(

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 def _config(g=24, proposal=None, n=64, steps=100, upsamp=(), density_shift=10.0):
     from tensorf_amd import training
     return training.TensorfConfig(density_shift=density_shift,
-        minibatch_size=1024, n_iters=steps, initial_aabb_min=(-2.0,) * 3,
+        minibatch_size=2048, n_iters=steps, initial_aabb_min=(-2.0,) * 3,
         initial_aabb_max=(2.0,) * 3, appearance_feat_dim=48, density_feat_dim=16,
         feature_n_freqs=2, viewdir_n_freqs=2, grid_dim_init=g, grid_dim_final=g + 8,
         upsamp_iters=upsamp, density_samples_per_ray=None if proposal else n,
@@ -78,7 +78,7 @@ def test_checkpoint_roundtrip(tmp_path):
     torch.cuda.synchronize()
     # the loss and the factor gradients are summed with atomics (order-dependent
     # rounding): compare with tolerances
-    assert abs(st._mse.item() - st2._mse.item()) <= 1e-6 * st._mse.item()
+    assert abs(st._mse.item() - st2._mse.item()) <= 1e-5 * st._mse.item()
     for a, b in ((st.flat, st2.flat), (st.prop_flat, st2.prop_flat)):
         d = (a.params - b.params).abs()
         frac = (d > 1e-5).float().mean().item()
@@ -122,22 +122,22 @@ def test_proposal_beats_uniform_at_equal_samples():
     PSNR than the stratified sampler (the method without it). The field starts
     transparent (density shift -10, TensoRF's original) so that geometry, not a
     view-dependent shell at the box boundary, is learned."""
-    log("case: proposal vs uniform, 32 field samples, 300 steps")
+    log("case: proposal vs uniform, 32 field samples, 600 steps")
     from tensorf_amd import proposal, training, utils
-    _, _, data = _small_dataset(views=16, res=48)
+    _, _, data = _small_dataset(views=16, res=64)
     res = {}
     for name, pc in (("uniform", None),
                      ("proposal", proposal.ProposalConfig(samples_round0=64, samples_round1=32,
                                                           samples_out=32, grid_dim=32,
-                                                          anneal_steps=100))):
-        cfg = _config(g=48, proposal=pc, n=32, steps=300, density_shift=-10.0)
-        st = training.TrainState.initialize(cfg, 48, prng_key=3, num_cameras=16)
-        batches = data.minibatches(1024, seed=1)
+                                                          anneal_steps=150))):
+        cfg = _config(g=64, proposal=pc, n=32, steps=600, density_shift=-10.0)
+        st = training.TrainState.initialize(cfg, 64, prng_key=3, num_cameras=16)
+        batches = data.minibatches(2048, seed=1)
         mses = []
-        for it in range(300):
+        for it in range(600):
             st, logs = st.training_step(next(batches))
-            if it >= 250:
+            if it >= 550:
                 mses.append(logs["mse"].item())
         res[name] = utils.psnr_from_mse(float(np.mean(mses)))
         log(f"  {name}: training psnr over the last 50 steps {res[name]:.2f} dB")
-    assert res["proposal"] > res["uniform"] + 1.0
+    assert res["proposal"] > res["uniform"] + 2.0
