@@ -72,6 +72,54 @@ struct PackedB {
   }
 };
 
+// FP32 table T[j][k] (stride ld, zero padded) as a B operand: B(k, j) = T[j][k];
+// four consecutive k of one column are one 16-byte load.
+struct PackedB32 {
+  __amdgpu_buffer_rsrc_t r;
+  int ld;
+  __device__ __forceinline__ PackedB32(const float* t, int ld_, int rows)
+      : r(buffer_rsrc(t, rows * ld_ * 4)), ld(ld_) {}
+  __device__ __forceinline__ float4 load4(int k, int j) const {
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    const i32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (j * ld + k) * 4, 0, 0);
+    return __builtin_bit_cast(float4, v);
+  }
+};
+
+// FP32 strip with a permuted K order: within each chunk of 16, MFMA c (c < 4)
+// sums k = 16i + 4*kq + c over the four lane groups kq, so that a lane's four
+// A operands (row r, k = 16i + 4kq .. +3) are one 16-byte LDS read and its
+// four B operands one 16-byte buffer load. The sum over k is unchanged; only
+// the order of the FP32 accumulation differs from a k-ascending chain.
+// Requires K % 16 == 0, lda % 4 == 0 and a 16-byte aligned A.
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB32 b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  const int lane = lane_id();
+  const int r = lane & 15, kq = lane >> 4;
+  constexpr int NS = KMAX / 16;
+  for (int kc = 0; kc < K; kc += KMAX) {  // one pass for K <= KMAX
+    float4 bb[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+      if (kc + 16 * i < K) bb[i] = b.load4(kc + 16 * i + 4 * kq, col0 + r);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      if (kc + 16 * i < K) {
+#pragma unroll
+        for (int rt = 0; rt < NR; ++rt) {
+          const float4 a = *reinterpret_cast<const float4*>(A + (16 * rt + r) * lda + kc +
+                                                            16 * i + 4 * kq);
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bb[i].x, acc[rt], 0, 0, 0);
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bb[i].y, acc[rt], 0, 0, 0);
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, bb[i].z, acc[rt], 0, 0, 0);
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, bb[i].w, acc[rt], 0, 0, 0);
+        }
+      }
+    }
+  }
+}
+
 // acc[rt] += A(rows 16*rt.., K) * B(K, cols col0..col0+15), rt < NR; A from
 // LDS (row-major, lda), K <= KMAX. The whole B column strip is loaded into
 // registers first (KMAX/4 floats or KMAX/32 bf16x8 per lane) so its L2

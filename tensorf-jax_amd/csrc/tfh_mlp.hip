@@ -10,11 +10,13 @@
 //
 // A workgroup owns a tile of TM rows. Activations live in LDS (FP32,
 // row-major, padded strides); every layer is a set of 16x16 output tiles:
-//   FP32: v_mfma_f32_16x16x4_f32, weights read from global (L2-resident);
+//   FP32: v_mfma_f32_16x16x4_f32 with a permuted K order so that a lane's
+//         four consecutive A operands are one 16-byte LDS read and its four B
+//         operands one 16-byte load from an FP32 weight table;
 //   BF16: v_mfma_f32_16x16x32_bf16 with FP32 accumulation; activations are
-//         rounded to BF16 when loaded as operands, weights are packed once per
-//         call into BF16 [out][in] / [in][out] tables so each lane's 8
-//         operands are one 16-byte load.
+//         rounded to BF16 when loaded as operands.
+// Weights are packed once per call into zero-padded [out][in] (forward) and
+// [in][out] (input-gradient) tables of the operand precision, L2-resident.
 // The backward recomputes the forward of its tile and accumulates weight
 // gradients into a per-workgroup partial buffer (plain read-modify-write,
 // each partial tile owned by one wave), reduced across workgroups by
@@ -44,13 +46,13 @@ struct MlpDims {
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-// FP32 pads K to 16 with strides = 2 mod 32 floats (conflict-free scalar
-// reads); BF16 pads K to 32 with strides = 4 mod 32 (16-byte aligned rows,
-// conflict-free 128-bit reads).
+// FP32 pads K to 16, BF16 to 32; row strides are 4 mod 8 floats: 16-byte
+// aligned rows whose 128-bit reads by 8 consecutive rows hit 8 distinct bank
+// groups.
 inline MlpDims make_dims(const tfh_mlp* p, bool bf) {
   MlpDims d;
   const int kp = bf ? 32 : 16;
-  const int pad = bf ? 4 : 2;
+  const int pad = 4;
   d.F = (int)p->feat_dim;
   d.FP = round_up(d.F, kp);
   d.S = (int)p->squash_dim;
@@ -73,7 +75,7 @@ inline MlpDims make_dims(const tfh_mlp* p, bool bf) {
 // the kernels built for it fold every LDS offset and tile count.
 template <bool BF>
 __host__ __device__ constexpr MlpDims std_dims() {
-  const int kp = BF ? 32 : 16, pad = BF ? 4 : 2;
+  const int kp = BF ? 32 : 16, pad = 4;
   const int F = 144, S = 27, nf = 2, nv = 2;
   const int E = S + 3 + 2 * nf * S + 2 * nv * 3;
   const int FP = (F + kp - 1) / kp * kp, EP = (E + kp - 1) / kp * kp;
@@ -91,13 +93,13 @@ struct MlpW {  // FP32 parameters (device pointers)
   const float *w0, *w1, *b1, *w2, *b2, *w3, *b3, *embed;
 };
 
-// Operand tables built per call in the workspace.
+// Operand tables built per call in the workspace: forward tables [out][in]
+// and input-gradient tables [in][out], zero padded:
+//   f0 [32][FP], f1 [U][EP], f2 [U][U], f3 [16][U]
+//   g0 [FP][32], g1 [EP][U], g2 [U][U], g3 [U][32]
 struct MlpPack {
-  // FP32 mode: transposed weights for the input-gradient GEMMs
-  const float *w0t, *w1t, *w2t, *w3t;  // (S,F), (U,E), (U,U), (3,U)
-  // BF16 mode: forward tables [out][in] and backward tables [in][out]
-  const __bf16 *f0, *f1, *f2, *f3;     // [32][FP], [U][EP], [U][U], [16][U]
-  const __bf16 *g0, *g1, *g2, *g3;     // [FP][32], [EP][U], [U][U], [U][32]
+  const __bf16 *f0, *f1, *f2, *f3, *g0, *g1, *g2, *g3;      // BF16 mode
+  const float *ff0, *ff1, *ff2, *ff3, *fg0, *fg1, *fg2, *fg3;  // FP32 mode
 };
 
 // B operands of each GEMM, by precision.
@@ -109,16 +111,16 @@ struct Operands<false> {
   const MlpDims& d;
   const MlpW& w;
   const MlpPack& p;
-  __device__ GlobalB l0() const { return GlobalB(w.w0, d.F, d.S); }
-  __device__ GlobalB l1() const { return GlobalB(w.w1, d.E, kU); }
-  __device__ GlobalB l2() const { return GlobalB(w.w2, kU, kU); }
-  __device__ GlobalB l3() const { return GlobalB(w.w3, kU, 3); }
-  __device__ GlobalB g3() const { return GlobalB(p.w3t, 3, kU); }
-  __device__ GlobalB g2() const { return GlobalB(p.w2t, kU, kU); }
-  __device__ GlobalB g1() const { return GlobalB(p.w1t, kU, d.E); }
-  __device__ GlobalB g0() const { return GlobalB(p.w0t, d.S, d.F); }
-  __device__ int k0() const { return round_up(d.F, 4); }
-  __device__ int k3() const { return 4; }  // K of dZ3 @ W3^T (3 padded to 4)
+  __device__ PackedB32 l0() const { return PackedB32(p.ff0, d.FP, 32); }
+  __device__ PackedB32 l1() const { return PackedB32(p.ff1, d.EP, kU); }
+  __device__ PackedB32 l2() const { return PackedB32(p.ff2, kU, kU); }
+  __device__ PackedB32 l3() const { return PackedB32(p.ff3, kU, 16); }
+  __device__ PackedB32 g3() const { return PackedB32(p.fg3, 32, kU); }
+  __device__ PackedB32 g2() const { return PackedB32(p.fg2, kU, kU); }
+  __device__ PackedB32 g1() const { return PackedB32(p.fg1, kU, d.EP); }
+  __device__ PackedB32 g0() const { return PackedB32(p.fg0, 32, d.FP); }
+  __device__ int k0() const { return d.FP; }
+  __device__ int k3() const { return 16; }  // K of dZ3 @ W3^T (3 padded to 16)
 };
 
 template <>
@@ -346,22 +348,20 @@ __host__ __device__ constexpr GradPack make_pack(const MlpDims& d) {
   return p;
 }
 
-// Operand tables written by k_pack_weights. One thread per table entry;
-// entries outside the real matrix are zero.
+// Operand tables written by k_pack_weights (element type T: __bf16 or
+// float). One thread per table entry; entries outside the real matrix are zero.
+template <typename T>
 struct PackOut {
-  float *w0t, *w1t, *w2t, *w3t;
-  __bf16 *f0, *f1, *f2, *f3, *g0, *g1, *g2, *g3;
+  T *f0, *f1, *f2, *f3, *g0, *g1, *g2, *g3;
 };
 
 struct PackSizes {
-  int64_t t0, t1, t2, t3;      // FP32 transposes
-  int64_t f0, f1, f2, f3;      // BF16 forward tables
-  int64_t g0, g1, g2, g3;      // BF16 backward tables
+  int64_t f0, f1, f2, f3;      // forward tables
+  int64_t g0, g1, g2, g3;      // input-gradient tables
 };
 
 inline PackSizes pack_sizes(const MlpDims& d) {
   PackSizes s;
-  s.t0 = (int64_t)d.S * d.F; s.t1 = (int64_t)kU * d.E; s.t2 = kU * kU; s.t3 = 3 * kU;
   s.f0 = (int64_t)32 * d.FP; s.f1 = (int64_t)kU * d.EP; s.f2 = kU * kU; s.f3 = 16 * kU;
   s.g0 = (int64_t)d.FP * 32; s.g1 = (int64_t)d.EP * kU; s.g2 = kU * kU; s.g3 = kU * 32;
   return s;
@@ -371,40 +371,29 @@ __device__ __forceinline__ float wat(const float* w, int rows, int cols, int r, 
   return (r < rows && c < cols) ? w[r * cols + c] : 0.0f;
 }
 
-template <bool BF>
-__global__ void k_pack_weights(MlpDims d, MlpW w, PackOut o, PackSizes s) {
-  const int64_t total = BF ? s.f0 + s.f1 + s.f2 + s.f3 + s.g0 + s.g1 + s.g2 + s.g3
-                           : s.t0 + s.t1 + s.t2 + s.t3;
+template <typename T>
+__global__ void k_pack_weights(MlpDims d, MlpW w, PackOut<T> o, PackSizes s) {
+  const int64_t total = s.f0 + s.f1 + s.f2 + s.f3 + s.g0 + s.g1 + s.g2 + s.g3;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     int64_t f = e;
-    if (!BF) {
-      if (f < s.t0) { const int j = f / d.F, i = f % d.F; o.w0t[f] = w.w0[i * d.S + j]; continue; }
-      f -= s.t0;
-      if (f < s.t1) { const int j = f / d.E, i = f % d.E; o.w1t[f] = w.w1[i * kU + j]; continue; }
-      f -= s.t1;
-      if (f < s.t2) { const int j = f / kU, i = f % kU; o.w2t[f] = w.w2[i * kU + j]; continue; }
-      f -= s.t2;
-      { const int j = f / kU, i = f % kU; o.w3t[f] = w.w3[i * 3 + j]; }
-      continue;
-    }
     // forward tables T[j][k] = W[k][j]
-    if (f < s.f0) { const int j = f / d.FP, k = f % d.FP; o.f0[f] = (__bf16)wat(w.w0, d.F, d.S, k, j); continue; }
+    if (f < s.f0) { const int j = f / d.FP, k = f % d.FP; o.f0[f] = (T)wat(w.w0, d.F, d.S, k, j); continue; }
     f -= s.f0;
-    if (f < s.f1) { const int j = f / d.EP, k = f % d.EP; o.f1[f] = (__bf16)wat(w.w1, d.E, kU, k, j); continue; }
+    if (f < s.f1) { const int j = f / d.EP, k = f % d.EP; o.f1[f] = (T)wat(w.w1, d.E, kU, k, j); continue; }
     f -= s.f1;
-    if (f < s.f2) { const int j = f / kU, k = f % kU; o.f2[f] = (__bf16)wat(w.w2, kU, kU, k, j); continue; }
+    if (f < s.f2) { const int j = f / kU, k = f % kU; o.f2[f] = (T)wat(w.w2, kU, kU, k, j); continue; }
     f -= s.f2;
-    if (f < s.f3) { const int j = f / kU, k = f % kU; o.f3[f] = (__bf16)wat(w.w3, kU, 3, k, j); continue; }
+    if (f < s.f3) { const int j = f / kU, k = f % kU; o.f3[f] = (T)wat(w.w3, kU, 3, k, j); continue; }
     f -= s.f3;
-    // backward tables T[j][k] = W[j][k] (j = input unit, k = output unit)
-    if (f < s.g0) { const int j = f / 32, k = f % 32; o.g0[f] = (__bf16)wat(w.w0, d.F, d.S, j, k); continue; }
+    // input-gradient tables T[j][k] = W[j][k] (j = input unit, k = output unit)
+    if (f < s.g0) { const int j = f / 32, k = f % 32; o.g0[f] = (T)wat(w.w0, d.F, d.S, j, k); continue; }
     f -= s.g0;
-    if (f < s.g1) { const int j = f / kU, k = f % kU; o.g1[f] = (__bf16)wat(w.w1, d.E, kU, j, k); continue; }
+    if (f < s.g1) { const int j = f / kU, k = f % kU; o.g1[f] = (T)wat(w.w1, d.E, kU, j, k); continue; }
     f -= s.g1;
-    if (f < s.g2) { const int j = f / kU, k = f % kU; o.g2[f] = (__bf16)wat(w.w2, kU, kU, j, k); continue; }
+    if (f < s.g2) { const int j = f / kU, k = f % kU; o.g2[f] = (T)wat(w.w2, kU, kU, j, k); continue; }
     f -= s.g2;
-    { const int j = f / 32, k = f % 32; o.g3[f] = (__bf16)wat(w.w3, kU, 3, j, k); }
+    { const int j = f / 32, k = f % 32; o.g3[f] = (T)wat(w.w3, kU, 3, j, k); }
   }
 }
 
@@ -788,45 +777,43 @@ static MlpW to_w(const tfh_mlp* p) {
 // Workspace of the weight tables of one precision.
 static size_t pack_bytes(const MlpDims& d, bool bf) {
   PackSizes s = pack_sizes(d);
-  if (!bf) return align_up((size_t)(s.t0 + s.t1 + s.t2 + s.t3) * sizeof(float), 256);
+  const size_t el = bf ? sizeof(__bf16) : sizeof(float);
   size_t n = 0;
   for (int64_t v : {s.f0, s.f1, s.f2, s.f3, s.g0, s.g1, s.g2, s.g3})
-    n += align_up((size_t)v * sizeof(__bf16), 256);
+    n += align_up((size_t)v * el, 256);
   return n;
 }
 
-// Carves and fills the weight tables; `need_fp32_t` selects the FP32 transposes.
-static int32_t build_pack(const tfh_mlp* p, const MlpDims& d, bool bf, bool need_fp32_t,
-                          Carve& cv, MlpPack& pk, hipStream_t st) {
+template <typename T>
+static PackOut<T> carve_tables(Carve& cv, const PackSizes& s) {
+  PackOut<T> o;
+  o.f0 = cv.take<T>(s.f0); o.f1 = cv.take<T>(s.f1); o.f2 = cv.take<T>(s.f2);
+  o.f3 = cv.take<T>(s.f3); o.g0 = cv.take<T>(s.g0); o.g1 = cv.take<T>(s.g1);
+  o.g2 = cv.take<T>(s.g2); o.g3 = cv.take<T>(s.g3);
+  return o;
+}
+
+// Carves and fills the weight tables of the operand precision.
+static int32_t build_pack(const tfh_mlp* p, const MlpDims& d, bool bf, Carve& cv, MlpPack& pk,
+                          hipStream_t st) {
   pk = MlpPack{};
   PackSizes s = pack_sizes(d);
-  PackOut o{};
   if (bf) {
-    o.f0 = cv.take<__bf16>(s.f0); o.f1 = cv.take<__bf16>(s.f1);
-    o.f2 = cv.take<__bf16>(s.f2); o.f3 = cv.take<__bf16>(s.f3);
-    o.g0 = cv.take<__bf16>(s.g0); o.g1 = cv.take<__bf16>(s.g1);
-    o.g2 = cv.take<__bf16>(s.g2); o.g3 = cv.take<__bf16>(s.g3);
-  } else if (need_fp32_t) {
-    o.w0t = cv.take<float>(s.t0 + s.t1 + s.t2 + s.t3);
-    if (o.w0t) {
-      o.w1t = o.w0t + s.t0;
-      o.w2t = o.w1t + s.t1;
-      o.w3t = o.w2t + s.t2;
-    }
+    PackOut<__bf16> o = carve_tables<__bf16>(cv, s);
+    if (!cv.ok) return fail(TFH_ERR_WORKSPACE, "mlp: workspace too small for the weight tables");
+    ProfScope ps("mlp_pack", st);
+    hipLaunchKernelGGL(k_pack_weights<__bf16>, dim3(128), dim3(256), 0, st, d, to_w(p), o, s);
+    pk.f0 = o.f0; pk.f1 = o.f1; pk.f2 = o.f2; pk.f3 = o.f3;
+    pk.g0 = o.g0; pk.g1 = o.g1; pk.g2 = o.g2; pk.g3 = o.g3;
   } else {
-    return TFH_OK;
+    PackOut<float> o = carve_tables<float>(cv, s);
+    if (!cv.ok) return fail(TFH_ERR_WORKSPACE, "mlp: workspace too small for the weight tables");
+    ProfScope ps("mlp_pack", st);
+    hipLaunchKernelGGL(k_pack_weights<float>, dim3(128), dim3(256), 0, st, d, to_w(p), o, s);
+    pk.ff0 = o.f0; pk.ff1 = o.f1; pk.ff2 = o.f2; pk.ff3 = o.f3;
+    pk.fg0 = o.g0; pk.fg1 = o.g1; pk.fg2 = o.g2; pk.fg3 = o.g3;
   }
-  if (!cv.ok) return fail(TFH_ERR_WORKSPACE, "mlp: workspace too small for the weight tables");
-  ProfScope ps("mlp_pack", st);
-  if (bf)
-    hipLaunchKernelGGL(k_pack_weights<true>, dim3(128), dim3(256), 0, st, d, to_w(p), o, s);
-  else
-    hipLaunchKernelGGL(k_pack_weights<false>, dim3(64), dim3(256), 0, st, d, to_w(p), o, s);
-  TFH_TRY(check_launch("mlp: weight tables"));
-  pk.w0t = o.w0t; pk.w1t = o.w1t; pk.w2t = o.w2t; pk.w3t = o.w3t;
-  pk.f0 = o.f0; pk.f1 = o.f1; pk.f2 = o.f2; pk.f3 = o.f3;
-  pk.g0 = o.g0; pk.g1 = o.g1; pk.g2 = o.g2; pk.g3 = o.g3;
-  return TFH_OK;
+  return check_launch("mlp: weight tables");
 }
 
 // Dynamic LDS above 64 KB needs the per-kernel opt-in.
@@ -841,7 +828,7 @@ static void allow_lds(K kernel, size_t bytes) {
 
 size_t mlp_fwd_workspace(const tfh_mlp* p, int32_t precision) {
   const bool bf = precision == TFH_PREC_BF16;
-  return bf ? pack_bytes(make_dims(p, true), true) : 0;
+  return pack_bytes(make_dims(p, bf), bf);
 }
 
 int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
@@ -851,7 +838,7 @@ int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   MlpDims d = make_dims(p, bf);
   Carve cv(ws, ws_bytes);
   MlpPack pk;
-  TFH_TRY(build_pack(p, d, bf, false, cv, pk, st));
+  TFH_TRY(build_pack(p, d, bf, cv, pk, st));
   const size_t lds = fwd_lds_floats<kFwdTM>(d) * sizeof(float);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp fwd: tile needs %zu bytes of LDS", lds);
   const int64_t blocks = ceil_div(M, kFwdTM);
@@ -895,7 +882,7 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   const int64_t nblk = bwd_blocks(M);
   Carve cv(ws, ws_bytes);
   MlpPack pk;
-  TFH_TRY(build_pack(p, d, bf, true, cv, pk, st));
+  TFH_TRY(build_pack(p, d, bf, cv, pk, st));
   float* partials = cv.take<float>(nblk * gp.total);
   if (!cv.ok)
     return fail(TFH_ERR_WORKSPACE, "mlp bwd: workspace of %zu bytes, need %zu", ws_bytes,
