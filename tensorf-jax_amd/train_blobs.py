@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--gt-samples", type=int, default=4096)
     ap.add_argument("--log-every", type=int, default=500)
     ap.add_argument("--chunk", type=int, default=65536)
+    ap.add_argument("--eval-jitter", type=int, default=1,
+                    help="1: render the test views with jittered samples (prng key 0), "
+                         "0: deterministic sample positions")
     ap.add_argument("--checkpoint", default="")
     ap.add_argument("--out", default="")
     args = ap.parse_args()
@@ -123,6 +126,7 @@ def main():
     rgb_cfg = state.render_config()
     depth_cfg = state.render_config(render.RenderMode.DIST_MEDIAN)
     psnrs, depth_err, depth_miss = [], [], []
+    eval_key = 0 if args.eval_jitter else None
     t_render = 0.0
     for i in test_ids:
         tf = focal * args.test_res / args.res
@@ -131,10 +135,12 @@ def main():
         torch.cuda.synchronize()
         tr = time.perf_counter()
         pred = render.render_rays_batched(state.appearance_mlp, state.learnable_params,
-                                          state.aabb, rays, 0, rgb_cfg, batch_size=args.chunk)
+                                          state.aabb, rays, eval_key, rgb_cfg,
+                                          batch_size=args.chunk)
         t_render += time.perf_counter() - tr
         dep = render.render_rays_batched(state.appearance_mlp, state.learnable_params,
-                                         state.aabb, rays, 0, depth_cfg, batch_size=args.chunk)
+                                         state.aabb, rays, eval_key, depth_cfg,
+                                         batch_size=args.chunk)
         pred = torch.tensor(pred, device=gt.device)
         psnrs.append(utils.psnr_from_mse(float(((pred - gt) ** 2).mean())))
         gd = gd.cpu().numpy()
@@ -149,14 +155,14 @@ def main():
                                 focal * args.test_res / args.res, 0)
     gt, _ = scenes.render_ground_truth(scene, rays, args.gt_samples)
     pred = torch.tensor(render.render_rays_batched(state.appearance_mlp, state.learnable_params,
-                                                   state.aabb, rays, 0, rgb_cfg,
+                                                   state.aabb, rays, eval_key, rgb_cfg,
                                                    batch_size=args.chunk), device=gt.device)
     train_view_psnr = utils.psnr_from_mse(float(((pred - gt) ** 2).mean()))
     log(f"training view 0: psnr={train_view_psnr:.3f}")
     res = {
         "sampler": args.sampler, "precision": args.precision, "rays_per_step": args.rays,
         "train_view_psnr": round(train_view_psnr, 4),
-        "density_shift": args.density_shift,
+        "density_shift": args.density_shift, "eval_jitter": bool(args.eval_jitter),
         "samples_per_ray": n, "appearance_samples_per_ray": k,
         "proposal_rounds": args.prop_rounds if pc else None, "steps": args.steps,
         "train_views": len(train_c2ws), "res": args.res, "grid": [args.grid_init, args.grid_final],
