@@ -112,21 +112,21 @@ def cpu_baseline(R_sample, N, k, G, cd, ca):
                       f"{dt:.1f} s"}
 
 
-def bench_proposal(args, cd, ca, G):
+def bench_proposal(args, cd, ca, G, n_out=64):
     """Proposal-sampled training and rendering: R rays, proposal rounds 128 ->
-    64 on a 64^3 x 8-channel proposal field, 64 field samples per ray on the
-    proposal's intervals, all 64 shaded by the BF16 feature MLP, Adam on both
+    64 on a 64^3 x 8-channel proposal field, n_out field samples per ray on the
+    proposal's intervals, all shaded by the BF16 feature MLP, Adam on both
     parameter sets. Returns the numbers merged into the JSON line."""
     import torch
     from tensorf_amd import _lib, cameras, proposal, render, training
     R = args.prop_rays
-    pc = proposal.ProposalConfig(samples_round0=128, samples_round1=64, samples_out=64)
+    pc = proposal.ProposalConfig(samples_round0=128, samples_round1=64, samples_out=n_out)
     cfg = training.TensorfConfig(
         minibatch_size=R, n_iters=30000, initial_aabb_min=(-1.5,) * 3,
         initial_aabb_max=(1.5,) * 3, appearance_feat_dim=ca, density_feat_dim=cd,
         feature_n_freqs=2, viewdir_n_freqs=2, grid_dim_init=G, grid_dim_final=G, upsamp_iters=(),
-        appearance_samples_per_ray=64, mlp_precision="bf16", proposal=pc)
-    progress(f"proposal: init state G={G} R={R} 128->64->64 bf16")
+        appearance_samples_per_ray=n_out, mlp_precision="bf16", proposal=pc)
+    progress(f"proposal: init state G={G} R={R} 128->64->{n_out} bf16")
     state = training.TrainState.initialize(cfg, G, prng_key=0, num_cameras=1)
     batches = make_batches(R, 4, seed=5, device="cuda")
     cams = torch.zeros(R, dtype=torch.int32, device="cuda")
@@ -174,8 +174,9 @@ def bench_proposal(args, cd, ca, G):
         stages = {name: round(ms / max(cnt, 1), 4) for name, (ms, cnt) in rec.items()}
         progress("proposal stages: " + ", ".join(f"{k}={v:.3f}ms" for k, v in stages.items()))
     return {"workload": f"proposal-sampled training step: R={R} rays, proposal 64^3 x 8 "
-                        f"channels + 2x64 FP32 MLP, rounds 128 -> 64, 64 field samples on the "
-                        f"proposal intervals all shaded by the BF16 MLP, VM {G}^3, Adam on both",
+                        f"channels + 2x64 FP32 MLP, rounds 128 -> 64, {n_out} field samples on "
+                        f"the proposal intervals all shaded by the BF16 MLP, VM {G}^3, Adam on "
+                        f"both",
             "train_rays_per_s": round(train, 1), "ms_per_step": round(1e3 * dt / args.steps, 3),
             "render_rays_per_s": round(rend, 1), "stages_ms": stages,
             "final_mse": float(state._mse.item())}
@@ -305,13 +306,15 @@ def main():
                         "traffic": traffic, "avg_ms": round(avg, 4),
                         "algorithmic_per_launch": amount}
 
-    prop = None
+    prop = prop128 = None
     if not args.no_proposal:
         del state, mbs, batches, rays, lp, ws
         torch.cuda.empty_cache()
-        prop = bench_proposal(args, cd, ca, G)
-        prop["train_speedup_vs_this_config"] = round(prop["train_rays_per_s"] / value, 2)
-        prop["render_speedup_vs_this_config"] = round(prop["render_rays_per_s"] / render_value, 2)
+        prop = bench_proposal(args, cd, ca, G, 64)
+        prop128 = bench_proposal(args, cd, ca, G, 128)
+        for p in (prop, prop128):
+            p["train_speedup_vs_this_config"] = round(p["train_rays_per_s"] / value, 2)
+            p["render_speedup_vs_this_config"] = round(p["render_rays_per_s"] / render_value, 2)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -342,6 +345,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "proposal": prop,
+        "proposal_128": prop128,
     }
     print(json.dumps(line), flush=True)
 

@@ -133,7 +133,7 @@ __device__ __forceinline__ void prop_store_tile(float* part, int off, int rows, 
   }
 }
 
-__global__ void __launch_bounds__(kPNW * 64) k_prop_mlp_bwd(PropW w, const float* __restrict__ feats,
+__global__ void __launch_bounds__(kPNW * 64, 2) k_prop_mlp_bwd(PropW w, const float* __restrict__ feats,
                                                             const float* __restrict__ dz,
                                                             int64_t M, float* __restrict__ dfeat,
                                                             float* __restrict__ partials) {
@@ -162,6 +162,7 @@ __global__ void __launch_bounds__(kPNW * 64) k_prop_mlp_bwd(PropW w, const float
     prop_hidden(w, Fs, H1, H2);
     // dW3 (64 x 1) += H2^T dZ: row tile = wave
     mma_wgrad_acc<false, kPTM>(H2, kPLdH, 16 * wave, DZ, kPLdZ, 0, wacc[0]);
+    __builtin_amdgcn_sched_barrier(0);
     if (threadIdx.x == 0)
       for (int i = 0; i < kPTM; ++i) db3 += DZ[i * kPLdZ];
     __syncthreads();
@@ -183,8 +184,11 @@ __global__ void __launch_bounds__(kPNW * 64) k_prop_mlp_bwd(PropW w, const float
     __syncthreads();
     // dW2 (64 x 64) += H1^T dZ2: wave owns column tile `wave`, all 4 row tiles
 #pragma unroll
-    for (int s = 0; s < kPC2; ++s)
+    for (int s = 0; s < kPC2; ++s) {
       mma_wgrad_acc<false, kPTM>(H1, kPLdH, 16 * s, H2, kPLdH, 16 * wave, wacc[kPC3 + s]);
+      // keep the operand loads of later tiles from being hoisted
+      __builtin_amdgcn_sched_barrier(0);
+    }
     if (threadIdx.x < kPH)
       for (int i = 0; i < kPTM; ++i) db2 += H2[i * kPLdH + threadIdx.x];
     __syncthreads();
@@ -206,9 +210,11 @@ __global__ void __launch_bounds__(kPNW * 64) k_prop_mlp_bwd(PropW w, const float
     __syncthreads();
     // dW1 (32 x 64) += Fs^T dZ1: wave owns column tile `wave`, 2 row tiles
 #pragma unroll
-    for (int s = 0; s < kPC1; ++s)
+    for (int s = 0; s < kPC1; ++s) {
       mma_wgrad_acc<false, kPTM>(Fs, kPLdF, 16 * s, H1, kPLdH, 16 * wave,
                                  wacc[kPC3 + kPC2 + s]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     if (threadIdx.x < kPH)
       for (int i = 0; i < kPTM; ++i) db1 += H1[i * kPLdH + threadIdx.x];
     // dF = dZ1 W1^T (64 x F): 2 column tiles on waves 0, 1
@@ -271,6 +277,12 @@ static int prop_num_cus() {
   return n;
 }
 
+// Persistent backward: two workgroups per CU (the kernel's register budget
+// is sized for that occupancy).
+static int64_t prop_bwd_blocks(int64_t M) {
+  return std::max<int64_t>(1, std::min<int64_t>(ceil_div(M, kPTM), 2 * (int64_t)prop_num_cus()));
+}
+
 int32_t check_prop_mlp(const tfh_prop_mlp* p, const char* who) {
   TFH_REQUIRE(p != nullptr, "%s: null tfh_prop_mlp", who);
   TFH_REQUIRE(p->w1 && p->b1 && p->w2 && p->b2 && p->w3 && p->b3, "%s: null proposal weight", who);
@@ -293,7 +305,7 @@ int32_t prop_mlp_fwd(const tfh_prop_mlp* p, const float* feats, int64_t M, float
 }
 
 size_t prop_mlp_bwd_workspace(const tfh_prop_mlp* p, int64_t M) {
-  const int64_t nblk = std::max<int64_t>(1, std::min<int64_t>(ceil_div(M, kPTM), prop_num_cus()));
+  const int64_t nblk = prop_bwd_blocks(M);
   return align_up((size_t)nblk * prop_pack((int)p->feat_dim).total * sizeof(float), 256);
 }
 
@@ -301,7 +313,7 @@ int32_t prop_mlp_bwd(const tfh_prop_mlp* p, const float* feats, const float* dz,
                      float* dfeat, const tfh_prop_mlp_grad* g, void* ws, size_t ws_bytes,
                      hipStream_t st) {
   if (M == 0) return TFH_OK;
-  const int64_t nblk = std::max<int64_t>(1, std::min<int64_t>(ceil_div(M, kPTM), prop_num_cus()));
+  const int64_t nblk = prop_bwd_blocks(M);
   const PropPack pp = prop_pack((int)p->feat_dim);
   TFH_REQUIRE(ws_bytes >= prop_mlp_bwd_workspace(p, M), "proposal mlp bwd: workspace too small");
   float* partials = static_cast<float*>(ws);

@@ -1,10 +1,10 @@
 // Device-side VM-decomposition gather / scatter shared by the kernels of
 // tfh_vm.hip and tfh_render.hip.
 //
-// Mapping: one sample per aligned group of 16 lanes; lane `l16` of the group
-// owns channels c = l16 + 16*q (q < ceil(C/16)), so each tap of a sample is
-// one contiguous C*4-byte read (or atomic) per group in the channel-last
-// layout plane[m][row][col][c], line[m][v][c].
+// Mapping: one sample per aligned group of LPS lanes (16, or 8 for fields of
+// at most 8 channels); lane `l` of the group owns channels c = l + LPS*q, so
+// each tap of a sample is one contiguous C*4-byte read (or atomic) per group
+// in the channel-last layout plane[m][row][col][c], line[m][v][c].
 #pragma once
 
 #include "tfh_common.h"
@@ -52,38 +52,42 @@ __device__ __forceinline__ void tap_values(const float* __restrict__ plane,
   pv = ((a.w00 * t00 + a.w01 * t01) + a.w10 * t10) + a.w11 * t11;
 }
 
-// Adds `val` at dst[key*C + c] once per distinct `key` among the four 16-lane
-// groups of the wave (all lanes of the wave must call it together). Lanes
-// with active == false contribute nothing.
+// Adds `val` at dst[key*C + c] once per distinct `key` among the 64/LPS
+// groups of LPS lanes of the wave (all lanes of the wave must call it
+// together): each group sums the values of the groups with its key and only
+// the lowest such group issues the atomic. Lanes with active == false
+// contribute nothing.
+template <int LPS = 16>
 __device__ __forceinline__ void merged_atomic_add(float* __restrict__ dst, int C, int c, int key,
                                                   float val, bool active) {
-  const int grp = lane_id() >> 4;
+  const int grp = lane_id() / LPS;
   const int k = active ? key : -1 - grp;
   const float v = active ? val : 0.0f;
   float acc = v;
   bool leader = active;
 #pragma unroll
-  for (int o = 16; o < 64; o += 16) {
+  for (int o = LPS; o < 64; o += LPS) {
     const int pk = __shfl_xor(k, o, 64);
     const float pv = __shfl_xor(v, o, 64);
     if (pk == k) {
       acc += pv;
-      if ((grp ^ (o >> 4)) < grp) leader = false;
+      if ((grp ^ (o / LPS)) < grp) leader = false;
     }
   }
   if (leader && acc != 0.0f) unsafeAtomicAdd(dst + (size_t)k * C + c, acc);
 }
 
 // Scatter of one mode: dL/dline = gl * (per-tap weight), dL/dplane = gp * weight.
+template <int LPS = 16>
 __device__ __forceinline__ void tap_scatter(float* __restrict__ gplane, float* __restrict__ gline,
                                             int C, int c, const TapAddr& a, float gl, float gp,
                                             bool active) {
-  merged_atomic_add(gline, C, c, a.line0, a.wl0 * gl, active);
-  merged_atomic_add(gline, C, c, a.line1, a.wl1 * gl, active);
-  merged_atomic_add(gplane, C, c, a.p00, a.w00 * gp, active);
-  merged_atomic_add(gplane, C, c, a.p01, a.w01 * gp, active);
-  merged_atomic_add(gplane, C, c, a.p10, a.w10 * gp, active);
-  merged_atomic_add(gplane, C, c, a.p11, a.w11 * gp, active);
+  merged_atomic_add<LPS>(gline, C, c, a.line0, a.wl0 * gl, active);
+  merged_atomic_add<LPS>(gline, C, c, a.line1, a.wl1 * gl, active);
+  merged_atomic_add<LPS>(gplane, C, c, a.p00, a.w00 * gp, active);
+  merged_atomic_add<LPS>(gplane, C, c, a.p01, a.w01 * gp, active);
+  merged_atomic_add<LPS>(gplane, C, c, a.p10, a.w10 * gp, active);
+  merged_atomic_add<LPS>(gplane, C, c, a.p11, a.w11 * gp, active);
 }
 
 }  // namespace tfh

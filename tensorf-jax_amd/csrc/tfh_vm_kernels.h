@@ -36,14 +36,17 @@ struct RaySamples {
   }
 };
 
-constexpr int kGatherSamples = 16;  // samples per 256-thread block
+constexpr int kGatherThreads = 256;
 
-template <class Src, bool ROW_MAJOR>
-__global__ void __launch_bounds__(256) k_vm_gather(tfh_vm vm, Src src, int64_t M,
-                                                   float* __restrict__ out) {
-  __shared__ float tile[ROW_MAJOR ? 1 : 3 * 64][kGatherSamples + 1];
-  const int grp = threadIdx.x >> 4, l16 = threadIdx.x & 15;
-  const int64_t s0 = (int64_t)blockIdx.x * kGatherSamples;
+// Gather: LPS lanes per sample, 256/LPS samples per block.
+template <class Src, bool ROW_MAJOR, int LPS>
+__global__ void __launch_bounds__(kGatherThreads) k_vm_gather(tfh_vm vm, Src src, int64_t M,
+                                                              float* __restrict__ out) {
+  constexpr int SPB = kGatherThreads / LPS;
+  constexpr int NQ = 64 / LPS;  // channel slots per lane (channels <= 64)
+  __shared__ float tile[ROW_MAJOR ? 1 : 3 * 64][SPB + 1];
+  const int grp = threadIdx.x / LPS, l = threadIdx.x % LPS;
+  const int64_t s0 = (int64_t)blockIdx.x * SPB;
   const int64_t s = s0 + grp;
   const bool active = s < M;
   const int G = (int)vm.grid_dim, C = (int)vm.channels;
@@ -56,8 +59,8 @@ __global__ void __launch_bounds__(256) k_vm_gather(tfh_vm vm, Src src, int64_t M
     TapAddr a;
     tap_addrs(mt[m], m, G, a);
 #pragma unroll
-    for (int q = 0; q < kMaxQ; ++q) {
-      const int c = l16 + 16 * q;
+    for (int q = 0; q < NQ; ++q) {
+      const int c = l + LPS * q;
       if (c < C && active) {
         float lv, pv;
         tap_values(vm.plane, vm.line, C, c, a, lv, pv);
@@ -72,8 +75,8 @@ __global__ void __launch_bounds__(256) k_vm_gather(tfh_vm vm, Src src, int64_t M
   if (!ROW_MAJOR) {
     __syncthreads();
     const int nch = 3 * C;
-    for (int e = threadIdx.x; e < nch * kGatherSamples; e += 256) {
-      const int ch = e / kGatherSamples, j = e % kGatherSamples;
+    for (int e = threadIdx.x; e < nch * SPB; e += kGatherThreads) {
+      const int ch = e / SPB, j = e % SPB;
       if (s0 + j < M) out[(int64_t)ch * M + s0 + j] = tile[ch][j];
     }
   }
@@ -82,30 +85,38 @@ __global__ void __launch_bounds__(256) k_vm_gather(tfh_vm vm, Src src, int64_t M
 template <class Src, bool ROW_MAJOR>
 inline int32_t launch_vm_gather(const tfh_vm& vm, const Src& src, int64_t M, float* out,
                                 hipStream_t st) {
-  const int64_t blocks = ceil_div(M, kGatherSamples);
+  const bool narrow = vm.channels <= 8;
+  const int spb = kGatherThreads / (narrow ? 8 : 16);
+  const int64_t blocks = ceil_div(M, spb);
   TFH_REQUIRE(blocks < (1ll << 31), "vm gather: too many points (%lld)", (long long)M);
-  hipLaunchKernelGGL((k_vm_gather<Src, ROW_MAJOR>), dim3((unsigned)blocks), dim3(256), 0, st, vm,
-                     src, M, out);
+  if (narrow)
+    hipLaunchKernelGGL((k_vm_gather<Src, ROW_MAJOR, 8>), dim3((unsigned)blocks),
+                       dim3(kGatherThreads), 0, st, vm, src, M, out);
+  else
+    hipLaunchKernelGGL((k_vm_gather<Src, ROW_MAJOR, 16>), dim3((unsigned)blocks),
+                       dim3(kGatherThreads), 0, st, vm, src, M, out);
   return check_launch("vm gather");
 }
 
 // VJP: grad (per sample, per channel) -> atomic scatter into the factor
-// gradients. Each wave holds 4 samples; equal texels among them are merged
-// before the atomic (merged_atomic_add).
-template <class Src, bool ROW_MAJOR>
-__global__ void __launch_bounds__(256) k_vm_scatter(tfh_vm vm, Src src, int64_t M,
-                                                    const float* __restrict__ gout,
-                                                    tfh_vm_grad grad) {
-  __shared__ float tile[ROW_MAJOR ? 1 : 3 * 64][kGatherSamples + 1];
-  const int grp = threadIdx.x >> 4, l16 = threadIdx.x & 15;
-  const int64_t s0 = (int64_t)blockIdx.x * kGatherSamples;
+// gradients. Each wave holds 64/LPS consecutive samples; equal texels among
+// them are merged before the atomic (merged_atomic_add).
+template <class Src, bool ROW_MAJOR, int LPS>
+__global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src src, int64_t M,
+                                                               const float* __restrict__ gout,
+                                                               tfh_vm_grad grad) {
+  constexpr int SPB = kGatherThreads / LPS;
+  constexpr int NQ = 64 / LPS;
+  __shared__ float tile[ROW_MAJOR ? 1 : 3 * 64][SPB + 1];
+  const int grp = threadIdx.x / LPS, l = threadIdx.x % LPS;
+  const int64_t s0 = (int64_t)blockIdx.x * SPB;
   const int64_t s = s0 + grp;
   const bool active = s < M;
   const int G = (int)vm.grid_dim, C = (int)vm.channels;
   if (!ROW_MAJOR) {
     const int nch = 3 * C;
-    for (int e = threadIdx.x; e < nch * kGatherSamples; e += 256) {
-      const int ch = e / kGatherSamples, j = e % kGatherSamples;
+    for (int e = threadIdx.x; e < nch * SPB; e += kGatherThreads) {
+      const int ch = e / SPB, j = e % SPB;
       tile[ch][j] = (s0 + j < M) ? gout[(int64_t)ch * M + s0 + j] : 0.0f;
     }
     __syncthreads();
@@ -119,16 +130,16 @@ __global__ void __launch_bounds__(256) k_vm_scatter(tfh_vm vm, Src src, int64_t 
     TapAddr a;
     tap_addrs(mt[m], m, G, a);
 #pragma unroll
-    for (int q = 0; q < kMaxQ; ++q) {
-      const int c = l16 + 16 * q;
-      if (16 * q < C) {  // wave-uniform: every lane joins the merged atomics
+    for (int q = 0; q < NQ; ++q) {
+      const int c = l + LPS * q;
+      if (LPS * q < C) {  // wave-uniform: every lane joins the merged atomics
         const bool on = active && c < C;
         float lv = 0.f, pv = 0.f, g = 0.f;
         if (on) {
           tap_values(vm.plane, vm.line, C, c, a, lv, pv);
           g = ROW_MAJOR ? gout[s * 3 * C + m * C + c] : tile[m * C + c][grp];
         }
-        tap_scatter(grad.plane, grad.line, C, c, a, g * pv, g * lv, on);
+        tap_scatter<LPS>(grad.plane, grad.line, C, c, a, g * pv, g * lv, on);
       }
     }
   }
@@ -137,10 +148,16 @@ __global__ void __launch_bounds__(256) k_vm_scatter(tfh_vm vm, Src src, int64_t 
 template <class Src, bool ROW_MAJOR>
 inline int32_t launch_vm_scatter(const tfh_vm& vm, const Src& src, int64_t M, const float* gout,
                                  const tfh_vm_grad& grad, hipStream_t st) {
-  const int64_t blocks = ceil_div(M, kGatherSamples);
+  const bool narrow = vm.channels <= 8;
+  const int spb = kGatherThreads / (narrow ? 8 : 16);
+  const int64_t blocks = ceil_div(M, spb);
   TFH_REQUIRE(blocks < (1ll << 31), "vm scatter: too many points (%lld)", (long long)M);
-  hipLaunchKernelGGL((k_vm_scatter<Src, ROW_MAJOR>), dim3((unsigned)blocks), dim3(256), 0, st, vm,
-                     src, M, gout, grad);
+  if (narrow)
+    hipLaunchKernelGGL((k_vm_scatter<Src, ROW_MAJOR, 8>), dim3((unsigned)blocks),
+                       dim3(kGatherThreads), 0, st, vm, src, M, gout, grad);
+  else
+    hipLaunchKernelGGL((k_vm_scatter<Src, ROW_MAJOR, 16>), dim3((unsigned)blocks),
+                       dim3(kGatherThreads), 0, st, vm, src, M, gout, grad);
   return check_launch("vm scatter");
 }
 
