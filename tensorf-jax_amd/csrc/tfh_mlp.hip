@@ -200,23 +200,30 @@ __device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
 // Runs layers 0..2 of the tile. On exit: X holds the encoded input, H0 (if
 // non-null) a copy of h0, H1 = relu(z1), H2p = relu(z2) (pre-FiLM) and H2 the
 // layer-3 input (== H2p without embedding). Fs is read by layer 0 only.
-template <int TM, int NW, bool BF>
+// Work items are (column tile, group of RTI row tiles): RTI = TM/16 keeps a
+// column strip's B operands for all rows of the tile (few waves), RTI = 1
+// spreads a layer over more waves.
+template <int TM, int NW, bool BF, int RTI>
 __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
                                const float* Fs, float* X, float* H0, float* H1, float* H2p,
                                float* H2, const float* Ds, const int* Cs) {
-  constexpr int RT = TM / 16;
+  constexpr int RG = TM / 16 / RTI;  // row groups
+  // B prefetch depth (K per pass): the whole strip with few waves, 64 when the
+  // workgroup's waves share the register file (16 waves)
+  constexpr int K0 = RTI == 1 ? 48 : 192, K1 = RTI == 1 ? 64 : 160, K2 = RTI == 1 ? 64 : kU;
   const int wave = threadIdx.x >> 6;
   const int lane = lane_id();
   // layer 0: (TM x F) @ (F x S) -> X[:, 0:S] (2 column tiles of 16)
-  for (int ct = wave; ct < 2; ct += NW) {
-    f32x4 acc[RT];
+  for (int it = wave; it < 2 * RG; it += NW) {
+    const int ct = it / RG, r0 = 16 * RTI * (it % RG);
+    f32x4 acc[RTI];
     zero_acc(acc);
-    mma_strip<RT, 192>(Fs, d.ldF, op.k0(), op.l0(), 16 * ct, acc);
+    mma_strip<RTI, K0>(Fs + r0 * d.ldF, d.ldF, op.k0(), op.l0(), 16 * ct, acc);
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
+    for (int rt = 0; rt < RTI; ++rt)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = 16 * rt + 4 * (lane >> 4) + e, col = 16 * ct + (lane & 15);
+        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e, col = 16 * ct + (lane & 15);
         if (col < d.S) {
           X[row * d.ldE + col] = acc[rt][e];
           if (H0) H0[row * d.ldS + col] = acc[rt][e];
@@ -227,33 +234,35 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
   encode_tile<TM>(d, X, Ds);
   __syncthreads();
   // layer 1: (TM x EP) @ (EP x U) + b1, relu -> H1
-  for (int ct = wave; ct < kU / 16; ct += NW) {
-    f32x4 acc[RT];
+  for (int it = wave; it < (kU / 16) * RG; it += NW) {
+    const int ct = it / RG, r0 = 16 * RTI * (it % RG);
+    f32x4 acc[RTI];
     zero_acc(acc);
-    mma_strip<RT, 160>(X, d.ldE, d.EP, op.l1(), 16 * ct, acc);
+    mma_strip<RTI, K1>(X + r0 * d.ldE, d.ldE, d.EP, op.l1(), 16 * ct, acc);
     const int col = 16 * ct + (lane & 15);
     const float bias = w.b1[col];
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
+    for (int rt = 0; rt < RTI; ++rt)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = 16 * rt + 4 * (lane >> 4) + e;
+        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
         H1[row * d.ldU + col] = fmaxf(acc[rt][e] + bias, 0.0f);
       }
   }
   __syncthreads();
   // layer 2: (TM x U) @ (U x U) + b2, relu (+ FiLM) -> H2p / H2
-  for (int ct = wave; ct < kU / 16; ct += NW) {
-    f32x4 acc[RT];
+  for (int it = wave; it < (kU / 16) * RG; it += NW) {
+    const int ct = it / RG, r0 = 16 * RTI * (it % RG);
+    f32x4 acc[RTI];
     zero_acc(acc);
-    mma_strip<RT, kU>(H1, d.ldU, kU, op.l2(), 16 * ct, acc);
+    mma_strip<RTI, K2>(H1 + r0 * d.ldU, d.ldU, kU, op.l2(), 16 * ct, acc);
     const int col = 16 * ct + (lane & 15);
     const float bias = w.b2[col];
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
+    for (int rt = 0; rt < RTI; ++rt)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = 16 * rt + 4 * (lane >> 4) + e;
+        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
         const float h = fmaxf(acc[rt][e] + bias, 0.0f);
         float out = h;
         if (d.ncam > 0 && col >= kU / 2) {
@@ -319,7 +328,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   __syncthreads();
   // layer 1 reads X (regB) and writes H1 over Fs (regA, dead after layer 0);
   // layer 2 reads H1 and writes H2 over X (dead after layer 1)
-  forward_hidden<TM, NW, BF>(d, w, op, regA, regB, nullptr, regA, regB, regB, Ds, Cs);
+  forward_hidden<TM, NW, BF, TM / 16>(d, w, op, regA, regB, nullptr, regA, regB, regB, Ds, Cs);
   forward_out<TM, NW, BF>(d, w, op, regB, R3);
   for (int e = threadIdx.x; e < TM * 3; e += blockDim.x) {
     const int i = e / 3, c = e % 3;
@@ -501,6 +510,10 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
                                                      float* __restrict__ partials,
                                                      float* __restrict__ gembed) {
   constexpr int RT = TM / 16;
+  // row tiles per work item of the strip phases: all of them (the B strip is
+  // loaded once per column tile) unless the workgroup has waves to spare
+  constexpr int BRTI = NW >= 16 ? 1 : RT;
+  constexpr int RG = RT / BRTI;
   const MlpDims d = STD ? std_dims<BF>() : d_arg;
   const GradPack gp = STD ? make_pack(std_dims<BF>()) : gp_arg;
   extern __shared__ float smem[];
@@ -533,7 +546,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     __syncthreads();
     load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
     __syncthreads();
-    forward_hidden<TM, NW, BF>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
+    forward_hidden<TM, NW, BF, BRTI>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
     forward_out<TM, NW, BF>(d, w, op, H2, Z3);
     // dZ3 = grgb * s * (1 - s), zero outside the 3 colours and the valid rows
     for (int e = threadIdx.x; e < TM * d.Z3W; e += blockDim.x) {
@@ -553,28 +566,29 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     if (threadIdx.x < 3)
       for (int i = 0; i < TM; ++i) db3 += Z3[i * d.ld3 + threadIdx.x];
     __syncthreads();
+    // Strip phases below run one work item per (column tile, row group).
     // dH2 = dZ3 @ W3^T; FiLM backward; dZ2 = dH2 * (h2 > 0) written over H2
-    for (int ct = wave; ct < kU / 16; ct += NW) {
-      f32x4 acc[RT];
+    for (int it = wave; it < (kU / 16) * RG; it += NW) {
+      const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
+      f32x4 acc[BRTI];
       zero_acc(acc);
-      mma_strip<RT, 32>(Z3, d.ld3, op.k3(), op.g3(), 16 * ct, acc);
+      mma_strip<BRTI, 32>(Z3 + r0 * d.ld3, d.ld3, op.k3(), op.g3(), 16 * ct, acc);
       const int col = 16 * ct + (lane & 15);
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int row = 16 * rt + 4 * (lane >> 4) + e;
-          float g = acc[rt][e];
-          const float hp = H2p[row * d.ldU + col];
-          if (d.ncam > 0 && col >= kU / 2 && row0 + row < M) {
-            const float* emb = w.embed + (size_t)Cs[row] * kU;
-            float* gemb = gembed + (size_t)Cs[row] * kU;
-            unsafeAtomicAdd(gemb + col - kU / 2, g * hp);
-            unsafeAtomicAdd(gemb + col, g);
-            g = g * emb[col - kU / 2];
-          }
-          H2[row * d.ldU + col] = hp > 0.0f ? g : 0.0f;
+      for (int q = 0; q < 4 * BRTI; ++q) {
+        const int rt = q / 4, e = q % 4;
+        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+        float g = acc[rt][e];
+        const float hp = H2p[row * d.ldU + col];
+        if (d.ncam > 0 && col >= kU / 2 && row0 + row < M) {
+          const float* emb = w.embed + (size_t)Cs[row] * kU;
+          float* gemb = gembed + (size_t)Cs[row] * kU;
+          unsafeAtomicAdd(gemb + col - kU / 2, g * hp);
+          unsafeAtomicAdd(gemb + col, g);
+          g = g * emb[col - kU / 2];
         }
+        H2[row * d.ldU + col] = hp > 0.0f ? g : 0.0f;
+      }
     }
     __syncthreads();
     // dW2 (U x U) += H1^T dZ2 ; db2
@@ -584,25 +598,18 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
       for (int i = 0; i < TM; ++i) db2 += H2[i * d.ldU + threadIdx.x];
     __syncthreads();
     // dH1 = dZ2 @ W2^T ; dZ1 = dH1 * (h1 > 0) written over H1 (dW2 is done)
-    {
-      constexpr int CT = (kU / 16 + NW - 1) / NW;
+    for (int it = wave; it < (kU / 16) * RG; it += NW) {
+      const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
+      f32x4 acc[BRTI];
+      zero_acc(acc);
+      mma_strip<BRTI, (BRTI == 1 ? 64 : kU)>(H2 + r0 * d.ldU, d.ldU, kU, op.g2(), 16 * ct, acc);
+      const int col = 16 * ct + (lane & 15);
 #pragma unroll
-      for (int s = 0; s < CT; ++s) {
-        const int ct = wave + s * NW;
-        if (ct < kU / 16) {
-          f32x4 acc[RT];
-          zero_acc(acc);
-          mma_strip<RT, kU>(H2, d.ldU, kU, op.g2(), 16 * ct, acc);
-          const int col = 16 * ct + (lane & 15);
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int row = 16 * rt + 4 * (lane >> 4) + e;
-              const float h = H1[row * d.ldU + col];
-              H1[row * d.ldU + col] = h > 0.0f ? acc[rt][e] : 0.0f;
-            }
-        }
+      for (int q = 0; q < 4 * BRTI; ++q) {
+        const int rt = q / 4, e = q % 4;
+        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+        const float h = H1[row * d.ldU + col];
+        H1[row * d.ldU + col] = h > 0.0f ? acc[rt][e] : 0.0f;
       }
     }
     __syncthreads();
@@ -616,21 +623,21 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     {
       const int fe0 = d.S + 3, fe1 = fe0 + 2 * d.nf * d.S;
       const int nct = d.EP / 16;
-      for (int ct = wave; ct < nct; ct += NW) {
+      for (int it = wave; it < nct * RG; it += NW) {
+        const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
         const int c0 = 16 * ct;
         const bool needed = (c0 < d.S) || (c0 + 16 > fe0 && c0 < fe1);
         if (!needed) continue;
-        f32x4 acc[RT];
+        f32x4 acc[BRTI];
         zero_acc(acc);
-        mma_strip<RT, kU>(H1, d.ldU, kU, op.g1(), c0, acc);
+        mma_strip<BRTI, (BRTI == 1 ? 64 : kU)>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
         const int col = c0 + (lane & 15);
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int row = 16 * rt + 4 * (lane >> 4) + e;
-            X[row * d.ldE + col] = acc[rt][e];  // dX (X is dead after dW1)
-          }
+        for (int q = 0; q < 4 * BRTI; ++q) {
+          const int rt = q / 4, e = q % 4;
+          const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+          X[row * d.ldE + col] = acc[rt][e];  // dX (X is dead after dW1)
+        }
       }
     }
     __syncthreads();
@@ -660,19 +667,19 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
                                                part, gp.w0, d.F, d.S);
     // dF = dH0 @ W0^T -> global
     if (gfeat) {
-      for (int ct = wave; ct < d.FP / 16; ct += NW) {
-        f32x4 acc[RT];
+      for (int it = wave; it < (d.FP / 16) * RG; it += NW) {
+        const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
+        f32x4 acc[BRTI];
         zero_acc(acc);
-        mma_strip<RT, kSP>(H0, d.ldS, kSP, op.g0(), 16 * ct, acc);
+        mma_strip<BRTI, kSP>(H0 + r0 * d.ldS, d.ldS, kSP, op.g0(), 16 * ct, acc);
         const int col = 16 * ct + (lane & 15);
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int row = 16 * rt + 4 * (lane >> 4) + e;
-            const int64_t grow = row0 + row;
-            if (grow < M && col < d.F) gfeat[grow * d.F + col] = acc[rt][e];
-          }
+        for (int q = 0; q < 4 * BRTI; ++q) {
+          const int rt = q / 4, e = q % 4;
+          const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+          const int64_t grow = row0 + row;
+          if (grow < M && col < d.F) gfeat[grow * d.F + col] = acc[rt][e];
+        }
       }
     }
   }
