@@ -182,6 +182,97 @@ def bench_proposal(args, cd, ca, G, n_out=64):
             "final_mse": float(state._mse.item())}
 
 
+def fisheye_batches(R, count, n_frames, seed, device):
+    """Minibatches of R rays drawn from equidistant-fisheye frames (960x540,
+    180 degree horizontal field) of cameras on a ring of radius 3 with +-0.3
+    height jitter; camera index = frame index; uniform random targets."""
+    import numpy as np
+    import torch
+    from tensorf_amd import cameras, scenes
+    c2ws = scenes.ring_cameras(n_frames, radius=3.0, height_jitter=0.3)
+    rng = np.random.default_rng(seed)
+    w, h = 960, 540
+    focal = (w / 2.0) / (np.pi / 2.0)
+    out = []
+    per = 4  # frames mixed per batch
+    for b in range(count):
+        os_, ds_, cs_ = [], [], []
+        for f in rng.choice(n_frames, per, replace=False):
+            rays = scenes.fisheye_rays(c2ws[f], w, h, focal, int(f), device)
+            flat = rays.reshape_flat()
+            sel = torch.tensor(rng.choice(w * h, R // per, replace=False), device=device)
+            os_.append(flat.origins[sel])
+            ds_.append(flat.directions[sel])
+            cs_.append(flat.camera_indices[sel])
+        colors = torch.tensor(rng.uniform(size=(R, 3)), dtype=torch.float32, device=device)
+        out.append(cameras.RenderedRays(colors, cameras.Rays3D(torch.cat(os_), torch.cat(ds_),
+                                                                torch.cat(cs_))))
+    return out
+
+
+def bench_leg(args, label, R, N, k, G, cd, ca, precision, contraction=False, n_frames=0):
+    """Uniform-sampled training and rendering on one further workload; returns
+    the numbers merged into the JSON line."""
+    import torch
+    from tensorf_amd import _lib, cameras, render, training
+    box = 2.0 if contraction else 1.5
+    cfg = training.TensorfConfig(
+        minibatch_size=R, n_iters=30000, initial_aabb_min=(-box,) * 3,
+        initial_aabb_max=(box,) * 3, appearance_feat_dim=ca, density_feat_dim=cd,
+        feature_n_freqs=2, viewdir_n_freqs=2, grid_dim_init=G, grid_dim_final=G, upsamp_iters=(),
+        scene_contraction=contraction, camera_embeddings=n_frames > 0,
+        density_samples_per_ray=N, appearance_samples_per_ray=k, mlp_precision=precision)
+    progress(f"{label}: init state G={G} R={R} N={N} k={k} {precision} "
+             f"contraction={contraction} cameras={n_frames}")
+    state = training.TrainState.initialize(cfg, G, prng_key=0, num_cameras=max(n_frames, 1))
+    if contraction:
+        mbs = fisheye_batches(R, 4, n_frames, seed=9, device="cuda")
+    else:
+        cams = torch.zeros(R, dtype=torch.int32, device="cuda")
+        mbs = [cameras.RenderedRays(c, cameras.Rays3D(o, d, cams))
+               for o, d, c in make_batches(R, 4, seed=7, device="cuda")]
+    steps = max(3, min(args.steps, 10))
+    for i in range(max(1, min(args.warmup, 3))):
+        state.training_step(mbs[i % 4])
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        state.training_step(mbs[i % 4])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    train = R * steps / dt
+    progress(f"{label}: {1e3 * dt / steps:.3f} ms/step, {train:.1f} rays/s")
+    lp = state.learnable_params
+    rcfg = state.render_config()
+    ws = state._ws
+    rays = [m.rays_wrt_world for m in mbs]
+
+    def render_once(i):
+        render._RenderEnv(state.appearance_mlp, lp, state.aabb, rays[i % 4], 7, rcfg,
+                          ws).forward()
+
+    render_once(0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(steps):
+        render_once(i)
+    torch.cuda.synchronize()
+    rend = R * steps / (time.perf_counter() - t1)
+    progress(f"{label} render: {rend:.1f} rays/s")
+    stages = {}
+    if not args.no_stages:
+        _lib.profile_enable(True)
+        for i in range(3):
+            state.training_step(mbs[i % 4])
+        torch.cuda.synchronize()
+        rec = _lib.profile_read()
+        _lib.profile_enable(False)
+        stages = {name: round(ms / max(cnt, 1), 4) for name, (ms, cnt) in rec.items()}
+    return {"train_rays_per_s": round(train, 1), "ms_per_step": round(1e3 * dt / steps, 3),
+            "render_rays_per_s": round(rend, 1), "stages_ms": stages,
+            "final_mse": float(state._mse.item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +290,8 @@ def main():
                     help="bytes of one plain device copy run first (calibrates HBM counters)")
     ap.add_argument("--prop-rays", type=int, default=16384)
     ap.add_argument("--no-proposal", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the top-k 64 BF16 and contracted fisheye workloads")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -306,10 +399,22 @@ def main():
                         "traffic": traffic, "avg_ms": round(avg, 4),
                         "algorithmic_per_launch": amount}
 
-    prop = prop128 = None
-    if not args.no_proposal:
-        del state, mbs, batches, rays, lp, ws
+    prop = prop128 = topk = unb = None
+    del state, mbs, batches, rays, lp, ws
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        topk = bench_leg(args, "topk64", 8192, 512, 64, G, cd, ca, "bf16")
+        topk["workload"] = (f"uniform training step: R=8192 rays x 512 stratified samples, "
+                            f"per-ray top-64 by weight shaded by the BF16 MLP, VM {G}^3")
         torch.cuda.empty_cache()
+        unb = bench_leg(args, "unbounded", 16384, 512, 512, G, cd, ca, "bf16",
+                        contraction=True, n_frames=200)
+        unb["workload"] = (f"unbounded training step: L-inf scene contraction, R=16384 "
+                           f"equidistant-fisheye rays (960x540 frames, 200 ring cameras, "
+                           f"FiLM camera embedding) x 512 samples uniform in contracted "
+                           f"distance, all shaded by the BF16 MLP, VM {G}^3")
+        torch.cuda.empty_cache()
+    if not args.no_proposal:
         prop = bench_proposal(args, cd, ca, G, 64)
         prop128 = bench_proposal(args, cd, ca, G, 128)
         for p in (prop, prop128):
@@ -346,6 +451,8 @@ def main():
         "cpu_baseline": cpu,
         "proposal": prop,
         "proposal_128": prop128,
+        "topk64_bf16": topk,
+        "unbounded_fisheye_bf16": unb,
     }
     print(json.dumps(line), flush=True)
 

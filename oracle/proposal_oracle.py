@@ -46,7 +46,7 @@ class ProposalConfig:
     samples_round1: int = 64
     samples_out: int = 64
     scene_contraction: bool = False
-    histogram_pad: float = 0.01
+    histogram_pad: float = 0.64
     loss_weight: float = 1.0
     weight_power: float = 1.0
 

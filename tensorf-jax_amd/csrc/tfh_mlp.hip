@@ -71,22 +71,23 @@ inline MlpDims make_dims(const tfh_mlp* p, bool bf) {
 }
 
 // Dims of the standard appearance MLP (48 channels x 3 modes, squash 27,
-// 2+2 Fourier frequencies, no camera embedding) as compile-time constants, so
-// the kernels built for it fold every LDS offset and tile count.
+// 2+2 Fourier frequencies) as compile-time constants, so the kernels built for
+// it fold every LDS offset and tile count; the camera count stays a run-time
+// value (0 = no embedding).
 template <bool BF>
-__host__ __device__ constexpr MlpDims std_dims() {
+__host__ __device__ constexpr MlpDims std_dims(int ncam = 0) {
   const int kp = BF ? 32 : 16, pad = 4;
   const int F = 144, S = 27, nf = 2, nv = 2;
   const int E = S + 3 + 2 * nf * S + 2 * nv * 3;
   const int FP = (F + kp - 1) / kp * kp, EP = (E + kp - 1) / kp * kp;
   const int Z3W = BF ? 32 : 16;
-  return MlpDims{F, FP, S, E, EP, nf, nv, 0, Z3W, FP + pad, EP + pad, 128 + pad, 32 + pad,
+  return MlpDims{F, FP, S, E, EP, nf, nv, ncam, Z3W, FP + pad, EP + pad, 128 + pad, 32 + pad,
                  Z3W + pad};
 }
 
 inline bool is_std_dims(const MlpDims& d, bool bf) {
   const MlpDims s = bf ? std_dims<true>() : std_dims<false>();
-  return d.F == s.F && d.S == s.S && d.nf == s.nf && d.nv == s.nv && d.ncam == 0;
+  return d.F == s.F && d.S == s.S && d.nf == s.nf && d.nv == s.nv;
 }
 
 struct MlpW {  // FP32 parameters (device pointers)
@@ -314,7 +315,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
                                                      const uint32_t* __restrict__ cams,
                                                      int64_t M, int64_t rpr,
                                                      float* __restrict__ rgb) {
-  const MlpDims d = STD ? std_dims<BF>() : d_arg;
+  const MlpDims d = STD ? std_dims<BF>(d_arg.ncam) : d_arg;
   extern __shared__ float smem[];
   // regions: A = Fs, later H1; B = X, later H2; small: Ds, R3, Cs
   float* regA = smem;
@@ -422,7 +423,8 @@ __device__ __forceinline__ void add_tile(float* __restrict__ part, int64_t off, 
 template <int TM>
 inline size_t bwd_lds_floats(const MlpDims& d) {
   return (size_t)TM * (d.ldF + d.ldE + d.ldS + 2 * d.ldU + (d.ncam > 0 ? d.ldU : 0) + d.ld3 + 4 +
-                       4);
+                       4) +
+         (d.ncam > 0 ? kU + 4 : 0);  // FiLM: running embedding gradient + 2 camera ids
 }
 
 // Weight-gradient tiles (16x16) of the four layers, numbered
@@ -500,7 +502,7 @@ constexpr int kCnt1 = (80 + kBwdNW - 1) / kBwdNW, kSlot1 = kSlot2 + kCnt2;
 constexpr int kCnt0 = (20 + kBwdNW - 1) / kBwdNW, kSlot0 = kSlot1 + kCnt1;
 constexpr int kSlots = kSlot0 + kCnt0;
 
-template <int TM, int NW, bool BF, int NT, bool STD>
+template <int TM, int NW, bool BF, int NT, bool STD, bool FILM>
 __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpPack pk, GradPack gp_arg,
                                                      const float* __restrict__ feats,
                                                      const float* __restrict__ dirs,
@@ -510,11 +512,13 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
                                                      float* __restrict__ partials,
                                                      float* __restrict__ gembed) {
   constexpr int RT = TM / 16;
+  static_assert(TM <= 64, "the FiLM camera check reads one tile row per lane");
   // row tiles per work item of the strip phases: all of them (the B strip is
   // loaded once per column tile) unless the workgroup has waves to spare
   constexpr int BRTI = NW >= 16 ? 1 : RT;
   constexpr int RG = RT / BRTI;
-  const MlpDims d = STD ? std_dims<BF>() : d_arg;
+  // the specialised shape folds the camera count to 0 without an embedding
+  const MlpDims d = STD ? std_dims<BF>(FILM ? d_arg.ncam : 0) : d_arg;
   const GradPack gp = STD ? make_pack(std_dims<BF>()) : gp_arg;
   extern __shared__ float smem[];
   float* Fs = smem;                         // TM x ldF
@@ -527,11 +531,21 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   float* Z3 = d.ncam > 0 ? after_h2 + TM * d.ldU : after_h2;  // TM x ld3: rgb, then dZ3
   float* Ds = Z3 + TM * d.ld3;
   int* Cs = reinterpret_cast<int*>(Ds + TM * 4);
+  // FiLM embedding gradient: tiles whose rows share one camera add their
+  // column sums into Eacc (g * h2p for the scale half, g for the shift half);
+  // Eacc goes to gembed with one atomic per column when the camera changes
+  // and at the end. Tiles with several cameras add per row directly.
+  float* Eacc = reinterpret_cast<float*>(Cs + 4 * TM);
+  int* Run = reinterpret_cast<int*>(Eacc + kU);  // [0] tile camera or -1, [1] camera of Eacc or -1
   const Operands<BF> op{d, w, pk};
 
   const int wave = threadIdx.x >> 6;
   const int lane = lane_id();
   float* part = partials + (int64_t)blockIdx.x * gp.total;
+  if (d.ncam > 0) {
+    for (int c = threadIdx.x; c < kU; c += blockDim.x) Eacc[c] = 0.0f;
+    if (threadIdx.x == 0) Run[1] = -1;
+  }
   const WTiles wt = wtiles(d);
   f32x4 wacc[NT > 0 ? NT : 1];
 #pragma unroll
@@ -546,6 +560,23 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     __syncthreads();
     load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
     __syncthreads();
+    if (d.ncam > 0 && wave == 0) {
+      // read before the next use: forward_hidden synchronises the workgroup
+      const int nvalid = (int)min((int64_t)TM, M - row0);
+      const int c0 = Cs[0];
+      const int tc = __all(lane >= nvalid || Cs[lane] == c0) ? c0 : -1;  // TM <= 64
+      const int rc = Run[1];
+      if (tc >= 0 && rc >= 0 && rc != tc) {
+        for (int c = lane; c < kU; c += 64) {
+          unsafeAtomicAdd(gembed + (size_t)rc * kU + c, Eacc[c]);
+          Eacc[c] = 0.0f;
+        }
+      }
+      if (lane == 0) {
+        Run[0] = tc;
+        if (tc >= 0) Run[1] = tc;
+      }
+    }
     forward_hidden<TM, NW, BF, BRTI>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
     forward_out<TM, NW, BF>(d, w, op, H2, Z3);
     // dZ3 = grgb * s * (1 - s), zero outside the 3 colours and the valid rows
@@ -568,26 +599,45 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     __syncthreads();
     // Strip phases below run one work item per (column tile, row group).
     // dH2 = dZ3 @ W3^T; FiLM backward; dZ2 = dH2 * (h2 > 0) written over H2
+    const int tcam = d.ncam > 0 ? Run[0] : -1;
     for (int it = wave; it < (kU / 16) * RG; it += NW) {
       const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
       f32x4 acc[BRTI];
       zero_acc(acc);
       mma_strip<BRTI, 32>(Z3 + r0 * d.ld3, d.ld3, op.k3(), op.g3(), 16 * ct, acc);
       const int col = 16 * ct + (lane & 15);
+      const bool film = d.ncam > 0 && 16 * ct >= kU / 2;  // wave-uniform
+      float sh = 0.0f, sg = 0.0f;
 #pragma unroll
       for (int q = 0; q < 4 * BRTI; ++q) {
         const int rt = q / 4, e = q % 4;
         const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
         float g = acc[rt][e];
         const float hp = H2p[row * d.ldU + col];
-        if (d.ncam > 0 && col >= kU / 2 && row0 + row < M) {
-          const float* emb = w.embed + (size_t)Cs[row] * kU;
-          float* gemb = gembed + (size_t)Cs[row] * kU;
-          unsafeAtomicAdd(gemb + col - kU / 2, g * hp);
-          unsafeAtomicAdd(gemb + col, g);
-          g = g * emb[col - kU / 2];
+        if (film) {
+          if (row0 + row < M) {
+            if (tcam >= 0) {
+              sh += g * hp;
+              sg += g;
+            } else {
+              float* gemb = gembed + (size_t)Cs[row] * kU;
+              unsafeAtomicAdd(gemb + col - kU / 2, g * hp);
+              unsafeAtomicAdd(gemb + col, g);
+            }
+          }
+          g = g * w.embed[(size_t)Cs[row] * kU + col - kU / 2];
         }
         H2[row * d.ldU + col] = hp > 0.0f ? g : 0.0f;
+      }
+      if (film && tcam >= 0) {
+        sh += __shfl_xor(sh, 16, 64);
+        sh += __shfl_xor(sh, 32, 64);
+        sg += __shfl_xor(sg, 16, 64);
+        sg += __shfl_xor(sg, 32, 64);
+        if (lane < 16) {
+          atomicAdd(Eacc + col - kU / 2, sh);  // LDS: row groups of one column may share it
+          atomicAdd(Eacc + col, sg);
+        }
       }
     }
     __syncthreads();
@@ -682,6 +732,13 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
         }
       }
     }
+  }
+  if (d.ncam > 0) {
+    __syncthreads();
+    const int rc = Run[1];
+    if (rc >= 0)
+      for (int c = threadIdx.x; c < kU; c += blockDim.x)
+        unsafeAtomicAdd(gembed + (size_t)rc * kU + c, Eacc[c]);
   }
   // the workgroup's gradient sums -> its partial
   if constexpr (NT > 0) {
@@ -906,17 +963,22 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
       return v && strcmp(v, "partial") == 0;
     }();
     const bool sd = is_std_dims(d, bf) && !force_partial;
-#define TFH_LAUNCH_BWD(BFV, NTV, STDV)                                                        \
-  do {                                                                                       \
-    allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV>, lds);                               \
-    hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV>), dim3((unsigned)nblk),    \
-                       dim3(kBwdNW * 64), lds, st, d, to_w(p), pk, gp, feats, dirs, cams,    \
-                       grgb, M, rpr, gfeat, partials, g->embed);                             \
+#define TFH_LAUNCH_BWD(BFV, NTV, STDV, FILMV)                                                 \
+  do {                                                                                        \
+    allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV, FILMV>, lds);                         \
+    hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV, FILMV>),                    \
+                       dim3((unsigned)nblk), dim3(kBwdNW * 64), lds, st, d, to_w(p), pk, gp,  \
+                       feats, dirs, cams, grgb, M, rpr, gfeat, partials, g->embed);           \
   } while (0)
+    const bool film = d.ncam > 0;
     if (bf) {
-      if (sd) TFH_LAUNCH_BWD(true, kBwdNT, true); else TFH_LAUNCH_BWD(true, 0, false);
+      if (sd && film) TFH_LAUNCH_BWD(true, kBwdNT, true, true);
+      else if (sd) TFH_LAUNCH_BWD(true, kBwdNT, true, false);
+      else TFH_LAUNCH_BWD(true, 0, false, true);
     } else {
-      if (sd) TFH_LAUNCH_BWD(false, kBwdNT, true); else TFH_LAUNCH_BWD(false, 0, false);
+      if (sd && film) TFH_LAUNCH_BWD(false, kBwdNT, true, true);
+      else if (sd) TFH_LAUNCH_BWD(false, kBwdNT, true, false);
+      else TFH_LAUNCH_BWD(false, 0, false, true);
     }
 #undef TFH_LAUNCH_BWD
   }

@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--appearance-samples", type=int, default=0, help="0 = all samples")
     ap.add_argument("--prop-rounds", default="128,64", help="proposal samples per round")
     ap.add_argument("--prop-out", type=int, default=64, help="field samples with the proposal")
-    ap.add_argument("--prop-pad", type=float, default=0.01, help="histogram pad mass")
+    ap.add_argument("--prop-pad", type=float, default=0.64, help="histogram pad mass")
     ap.add_argument("--prop-anneal", type=int, default=1000, help="weight-power ramp steps")
     ap.add_argument("--prop-grid", type=int, default=64)
     ap.add_argument("--rays", type=int, default=4096)

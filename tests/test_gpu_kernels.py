@@ -165,19 +165,35 @@ def test_topk_bit_exact(lib, n, k):
     assert (idx2 == want2).mean() > 0.999
 
 
-@pytest.mark.parametrize("ncam", [None, 5])
-def test_mlp_fwd_bwd(lib, ncam):
-    log(f"case: mlp fwd/bwd ncam={ncam}")
-    shape = orc.MlpShape(num_cameras=ncam)
-    p = oracle_params(4, 16, 48, shape, seed=3, bias_std=0.1)
+def _camera_layout(rng, ncam, m, layout):
+    """Per-row camera indices: independent per row, or runs of 70 rows cycling
+    through the cameras (32-row tiles with one camera and tiles that straddle
+    a change; a persistent workgroup meets several cameras in turn)."""
+    if layout == "runs":
+        return (np.arange(m) // 70) % ncam
+    return rng.integers(0, ncam or 1, size=m)
+
+
+# (cameras, camera layout, rows, appearance channels, Fourier frequencies):
+# 48 channels with 2+2 frequencies run the kernels specialised for that shape,
+# 24 channels with 6+6 (the reference's defaults) the generic ones
+MLP_CASES = [(None, "random", 1000, 48, 2), (5, "random", 1000, 48, 2),
+             (5, "runs", 20000, 48, 2), (None, "random", 1000, 24, 6), (3, "runs", 5000, 24, 6)]
+
+
+@pytest.mark.parametrize("ncam,layout,m,ca,nf", MLP_CASES)
+def test_mlp_fwd_bwd(lib, ncam, layout, m, ca, nf):
+    log(f"case: mlp fwd/bwd ncam={ncam} layout={layout} m={m} ca={ca} nf={nf}")
+    shape = orc.MlpShape(feat_dim=3 * ca, feature_n_freqs=nf, viewdir_n_freqs=nf,
+                         num_cameras=ncam)
+    p = oracle_params(4, 16, ca, shape, seed=3, bias_std=0.1)
     _, _, mp = to_device_params(p, shape)
     mlp = mlp_module(shape)
     rng = np.random.default_rng(11)
-    m = 1000
-    f = f32(rng.normal(0, 0.3, size=(m, 144)))
+    f = f32(rng.normal(0, 0.3, size=(m, 3 * ca)))
     d = rng.normal(size=(m, 3))
     d = f32(d / np.linalg.norm(d, axis=1, keepdims=True))
-    cams = rng.integers(0, ncam or 1, size=m)
+    cams = _camera_layout(rng, ncam, m, layout)
     for leaf in mlp.flat_leaves(mp):
         leaf.requires_grad_(True)
     tf = dev(f).requires_grad_(True)
@@ -224,23 +240,33 @@ def test_abi_errors(lib):
         lib.call("tfh_vm_interpolate", ctypes.byref(vm), None, 5, None, lib.stream())
 
 
-@pytest.mark.parametrize("ncam", [None, 5])
-def test_mlp_bf16_fwd_bwd(lib, ncam):
+@pytest.mark.parametrize("ncam,layout,m,ca,nf",
+                         [(None, "random", 777, 48, 2), (5, "random", 777, 48, 2)] + MLP_CASES[2:])
+def test_mlp_bf16_fwd_bwd(lib, ncam, layout, m, ca, nf):
     """BF16 operands on MFMA, FP32 accumulation. Checked against the oracle with
     the same BF16 operand rounding (tight: 2e-3 of the largest value, the FP32
     accumulation order and the rounding of near-ties differ), and reported
-    against float64 (bound: 2e-2 absolute on RGB)."""
-    log(f"case: mlp bf16 ncam={ncam}")
-    shape = orc.MlpShape(num_cameras=ncam)
-    p = oracle_params(4, 16, 48, shape, seed=3, bias_std=0.1)
+    against float64 (bound: 2e-2 absolute on RGB).
+
+    Gradients: every element within 2e-2 of the largest, except isolated
+    rounding-tie events. With 6 frequencies an encoded feature sin(32 h0) that
+    sits on a BF16 rounding tie rounds differently under FP32 and FP64
+    accumulation of h0, and the chain rule scales that by up to 32: the oracle
+    itself, FP32 against FP64 accumulation, differs by 0.174 (5% of the
+    largest feature gradient) at 2 of the 360,000 feature gradients of the
+    24-channel case. So at most 1e-3 of the elements may exceed 2e-2 of the
+    largest, and the relative L2 error must stay below 1e-2."""
+    log(f"case: mlp bf16 ncam={ncam} layout={layout} m={m} ca={ca} nf={nf}")
+    shape = orc.MlpShape(feat_dim=3 * ca, feature_n_freqs=nf, viewdir_n_freqs=nf,
+                         num_cameras=ncam)
+    p = oracle_params(4, 16, ca, shape, seed=3, bias_std=0.1)
     _, _, mp = to_device_params(p, shape)
     mlp = mlp_module(shape, "bf16")
     rng = np.random.default_rng(12)
-    m = 777
-    f = f32(rng.normal(0, 0.3, size=(m, 144)))
+    f = f32(rng.normal(0, 0.3, size=(m, 3 * ca)))
     d = rng.normal(size=(m, 3))
     d = f32(d / np.linalg.norm(d, axis=1, keepdims=True))
-    cams = rng.integers(0, ncam or 1, size=m)
+    cams = _camera_layout(rng, ncam, m, layout)
     for leaf in mlp.flat_leaves(mp):
         leaf.requires_grad_(True)
     tf = dev(f).requires_grad_(True)
@@ -253,9 +279,14 @@ def test_mlp_bf16_fwd_bwd(lib, ncam):
     g = f32(rng.normal(size=(m, 3)))
     (rgb * dev(g)).sum().backward()
     grads, gf = orc.feature_mlp_backward(p, cache, g, shape)
-    ef = report("grad features (bf16)", tf.grad.cpu().numpy(), gf)
-    assert ef.max() < 2e-2 * np.abs(gf).max()
+    def check(name, got_g, want_g):
+        e = report(name, got_g, want_g)
+        bound = 2e-2 * np.abs(want_g).max() + 1e-6
+        rel_l2 = np.linalg.norm(got_g - want_g) / max(np.linalg.norm(want_g), 1e-30)
+        assert (e > bound).mean() <= 1e-3, (name, (e > bound).sum())
+        assert rel_l2 < 1e-2, (name, rel_l2)
+
+    check("grad features (bf16)", tf.grad.cpu().numpy().astype(np.float64), gf)
     names = ["w0", "w1", "b1", "w2", "b2", "w3", "b3"] + (["embed"] if ncam else [])
     for name, leaf in zip(names, mlp.flat_leaves(mp)):
-        e = report(f"grad {name} (bf16)", leaf.grad.cpu().numpy(), grads[name])
-        assert e.max() < 2e-2 * np.abs(grads[name]).max() + 1e-6, name
+        check(f"grad {name} (bf16)", leaf.grad.cpu().numpy().astype(np.float64), grads[name])

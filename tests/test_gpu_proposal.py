@@ -62,7 +62,8 @@ def test_prop_mlp_fwd_bwd():
 
 
 @pytest.mark.parametrize("n,n_out,pad,power", [(37, 23, 0.01, 1.0), (128, 64, 0.0, 1.0),
-                                               (200, 300, 0.05, 0.5), (50, 20, 0.01, 0.0)])
+                                               (200, 300, 0.05, 0.5), (50, 20, 0.01, 0.0),
+                                               (64, 64, 0.64, 1.0)])
 def test_resample(n, n_out, pad, power):
     log(f"case: resample n={n} n_out={n_out} pad={pad} power={power}")
     from tensorf_amd import proposal
@@ -142,14 +143,15 @@ def _sample(pp, cfg, o, d, jit, contraction=False, base_ts=None, base_steps=None
     return s, field, box, keep
 
 
-@pytest.mark.parametrize("contraction,n0,n1,nout,power", [
-    (False, 32, 16, 16, 1.0), (True, 32, 16, 16, 1.0), (False, 128, 64, 64, 1.0),
-    (False, 128, 64, 64, 0.3), (True, 256, 96, 48, 1.0)])
-def test_proposal_sample(contraction, n0, n1, nout, power):
+@pytest.mark.parametrize("contraction,n0,n1,nout,power,pad", [
+    (False, 32, 16, 16, 1.0, 0.02), (True, 32, 16, 16, 1.0, 0.02),
+    (False, 128, 64, 64, 1.0, 0.02), (False, 128, 64, 64, 0.3, 0.02),
+    (True, 256, 96, 48, 1.0, 0.02), (False, 128, 64, 64, 1.0, 0.64)])
+def test_proposal_sample(contraction, n0, n1, nout, power, pad):
     log(f"case: proposal sample contraction={contraction} rounds {n0} -> {n1} -> {nout} "
-        f"power={power}")
+        f"power={power} pad={pad}")
     cfg = po.ProposalConfig(samples_round0=n0, samples_round1=n1, samples_out=nout,
-                            scene_contraction=contraction, histogram_pad=0.02,
+                            scene_contraction=contraction, histogram_pad=pad,
                             weight_power=power)
     pp = _prop_params()
     r = 96
