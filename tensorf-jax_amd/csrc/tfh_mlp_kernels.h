@@ -1,0 +1,825 @@
+// FeatureMlp kernels on gfx950 MFMA, shared by three translation units:
+// tfh_mlp.hip (forward, backward of the generic shapes), tfh_mlp_bwd_std.hip
+// (backward of the standard shape) and tfh_mlp_film.hip (the same with a FiLM
+// camera embedding). The standard backward sits at the 256-VGPR limit of two
+// waves per SIMD, and the compiler allocates its registers worse when other
+// instantiations share its translation unit: 17/31 spilled VGPRs (BF16/FP32)
+// on its own, 26/43 next to the generic kernels, 39/67 next to the FiLM one.
+//
+// FeatureMlp (networks.py:38-121).
+//
+// Layers (kernels are (in, out) like Flax Dense):
+//   h0 = f @ W0                       (F -> S, no bias)      networks.py:57-61
+//   x  = [h0, d, FE(h0, nf), FE(d, nv)]  (E wide)            networks.py:68-76
+//   h1 = relu(x @ W1 + b1)            (E -> U)               networks.py:86-90
+//   h2 = relu(h1 @ W2 + b2)           (U -> U)               networks.py:94-98
+//   h2[U/2:] = e[:U/2] * h2[U/2:] + e[U/2:],  e = embed[cam]  networks.py:103-111
+//   rgb = sigmoid(h2 @ W3 + b3)       (U -> 3)               networks.py:114-120
+//
+// A workgroup owns a tile of TM rows. Activations live in LDS (FP32,
+// row-major, padded strides); every layer is a set of 16x16 output tiles:
+//   FP32: v_mfma_f32_16x16x4_f32 with a permuted K order so that a lane's
+//         four consecutive A operands are one 16-byte LDS read and its four B
+//         operands one 16-byte load from an FP32 weight table;
+//   BF16: v_mfma_f32_16x16x32_bf16 with FP32 accumulation; activations are
+//         rounded to BF16 when loaded as operands.
+// Weights are packed once per call into zero-padded [out][in] (forward) and
+// [in][out] (input-gradient) tables of the operand precision, L2-resident.
+// The backward recomputes the forward of its tile and accumulates the weight
+// gradients in registers of their owning wave (standard shape) or in a
+// per-workgroup partial buffer (other shapes, plain read-modify-write, each
+// partial tile owned by one wave), reduced across workgroups by k_mlp_reduce.
+#pragma once
+
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "tfh_mfma.h"
+
+namespace tfh {
+
+constexpr int kU = 128;   // hidden width
+constexpr int kSP = 32;   // padded squash width
+
+struct MlpDims {
+  int F, FP;   // feature dim, padded
+  int S;       // squash dim (<= 32)
+  int E, EP;   // encoded dim, padded
+  int nf, nv;
+  int ncam;
+  int Z3W;     // padded width of the layer-3 output / its gradient
+  // LDS strides (floats)
+  int ldF, ldE, ldU, ldS, ld3;
+};
+
+__host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// FP32 pads K to 16, BF16 to 32; row strides are 4 mod 8 floats: 16-byte
+// aligned rows whose 128-bit reads by 8 consecutive rows hit 8 distinct bank
+// groups.
+inline MlpDims make_dims(const tfh_mlp* p, bool bf) {
+  MlpDims d;
+  const int kp = bf ? 32 : 16;
+  const int pad = 4;
+  d.F = (int)p->feat_dim;
+  d.FP = round_up(d.F, kp);
+  d.S = (int)p->squash_dim;
+  d.nf = (int)p->feature_n_freqs;
+  d.nv = (int)p->viewdir_n_freqs;
+  d.E = d.S + 3 + 2 * d.nf * d.S + 2 * d.nv * 3;
+  d.EP = round_up(d.E, kp);
+  d.ncam = p->embed ? (int)p->num_cameras : 0;
+  d.Z3W = bf ? 32 : 16;
+  d.ldF = d.FP + pad;
+  d.ldE = d.EP + pad;
+  d.ldU = kU + pad;
+  d.ldS = kSP + pad;
+  d.ld3 = d.Z3W + pad;
+  return d;
+}
+
+// Dims of the standard appearance MLP (48 channels x 3 modes, squash 27,
+// 2+2 Fourier frequencies) as compile-time constants, so the kernels built for
+// it fold every LDS offset and tile count; the camera count stays a run-time
+// value (0 = no embedding).
+template <bool BF>
+__host__ __device__ constexpr MlpDims std_dims(int ncam = 0) {
+  const int kp = BF ? 32 : 16, pad = 4;
+  const int F = 144, S = 27, nf = 2, nv = 2;
+  const int E = S + 3 + 2 * nf * S + 2 * nv * 3;
+  const int FP = (F + kp - 1) / kp * kp, EP = (E + kp - 1) / kp * kp;
+  const int Z3W = BF ? 32 : 16;
+  return MlpDims{F, FP, S, E, EP, nf, nv, ncam, Z3W, FP + pad, EP + pad, 128 + pad, 32 + pad,
+                 Z3W + pad};
+}
+
+inline bool is_std_dims(const MlpDims& d, bool bf) {
+  const MlpDims s = bf ? std_dims<true>() : std_dims<false>();
+  return d.F == s.F && d.S == s.S && d.nf == s.nf && d.nv == s.nv;
+}
+
+struct MlpW {  // FP32 parameters (device pointers)
+  const float *w0, *w1, *b1, *w2, *b2, *w3, *b3, *embed;
+};
+
+// Operand tables built per call in the workspace: forward tables [out][in]
+// and input-gradient tables [in][out], zero padded:
+//   f0 [32][FP], f1 [U][EP], f2 [U][U], f3 [16][U]
+//   g0 [FP][32], g1 [EP][U], g2 [U][U], g3 [U][32]
+struct MlpPack {
+  const __bf16 *f0, *f1, *f2, *f3, *g0, *g1, *g2, *g3;      // BF16 mode
+  const float *ff0, *ff1, *ff2, *ff3, *fg0, *fg1, *fg2, *fg3;  // FP32 mode
+};
+
+// B operands of each GEMM, by precision.
+template <bool BF>
+struct Operands;
+
+template <>
+struct Operands<false> {
+  const MlpDims& d;
+  const MlpW& w;
+  const MlpPack& p;
+  __device__ PackedB32 l0() const { return PackedB32(p.ff0, d.FP, 32); }
+  __device__ PackedB32 l1() const { return PackedB32(p.ff1, d.EP, kU); }
+  __device__ PackedB32 l2() const { return PackedB32(p.ff2, kU, kU); }
+  __device__ PackedB32 l3() const { return PackedB32(p.ff3, kU, 16); }
+  __device__ PackedB32 g3() const { return PackedB32(p.fg3, 32, kU); }
+  __device__ PackedB32 g2() const { return PackedB32(p.fg2, kU, kU); }
+  __device__ PackedB32 g1() const { return PackedB32(p.fg1, kU, d.EP); }
+  __device__ PackedB32 g0() const { return PackedB32(p.fg0, 32, d.FP); }
+  __device__ int k0() const { return d.FP; }
+  __device__ int k3() const { return 16; }  // K of dZ3 @ W3^T (3 padded to 16)
+};
+
+template <>
+struct Operands<true> {
+  const MlpDims& d;
+  const MlpW& w;
+  const MlpPack& p;
+  __device__ PackedB l0() const { return PackedB(p.f0, d.FP, 32); }
+  __device__ PackedB l1() const { return PackedB(p.f1, d.EP, kU); }
+  __device__ PackedB l2() const { return PackedB(p.f2, kU, kU); }
+  __device__ PackedB l3() const { return PackedB(p.f3, kU, 16); }
+  __device__ PackedB g3() const { return PackedB(p.g3, 32, kU); }
+  __device__ PackedB g2() const { return PackedB(p.g2, kU, kU); }
+  __device__ PackedB g1() const { return PackedB(p.g1, kU, d.EP); }
+  __device__ PackedB g0() const { return PackedB(p.g0, 32, d.FP); }
+  __device__ int k0() const { return d.FP; }
+  __device__ int k3() const { return 32; }
+};
+
+// ---- shared forward pieces ------------------------------------------------------
+// Loads TM rows of features (zero padded to FP columns and past M), view
+// directions and camera indices of the tile. Row `row` belongs to ray
+// row / rpr (view direction and camera index are per ray).
+template <int TM>
+__device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
+                          const float* __restrict__ dirs, const uint32_t* __restrict__ cams,
+                          int64_t row0, int64_t M, int64_t rpr, float* Fs, float* Ds,
+                          int* Cs) {
+  for (int e = threadIdx.x; e < TM * d.FP; e += blockDim.x) {
+    const int i = e / d.FP, c = e % d.FP;
+    const int64_t row = row0 + i;
+    Fs[i * d.ldF + c] = (row < M && c < d.F) ? feats[row * d.F + c] : 0.0f;
+  }
+  for (int e = threadIdx.x; e < TM * 3; e += blockDim.x) {
+    const int i = e / 3, c = e % 3;
+    const int64_t row = row0 + i;
+    Ds[i * 3 + c] = row < M ? dirs[(row / rpr) * 3 + c] : 0.0f;
+  }
+  for (int i = threadIdx.x; i < TM; i += blockDim.x) {
+    const int64_t row = row0 + i;
+    int cam = (cams && row < M) ? (int)cams[row / rpr] : 0;
+    if (cam < 0 || cam >= d.ncam) cam = 0;
+    Cs[i] = cam;
+  }
+}
+
+// Fills X[:, S:EP] = [d, FE(h0, nf), FE(d, nv), 0...] from h0 = X[:, 0:S]
+// (networks.py:13-35, 68-76): FE(v, n)[j*2n + q] = sin(2^q v_j) for q < n and
+// sin(2^(q-n) v_j + pi/2) otherwise.
+template <int TM>
+__device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
+  const int ncol = d.EP - d.S;
+  const int fe0 = d.S + 3;
+  const int fe1 = fe0 + 2 * d.nf * d.S;
+  const int fe2 = fe1 + 2 * d.nv * 3;
+  for (int e = threadIdx.x; e < TM * ncol; e += blockDim.x) {
+    const int i = e / ncol, c = d.S + e % ncol;
+    float v;
+    if (c < fe0) {
+      v = Ds[i * 3 + (c - d.S)];
+    } else if (c < fe2) {
+      const bool feat = c < fe1;
+      const int t = feat ? c - fe0 : c - fe1;
+      const int n = feat ? d.nf : d.nv;
+      const int j = t / (2 * n), q = t % (2 * n);
+      const float src = feat ? X[i * d.ldE + j] : Ds[i * 3 + j];
+      const float scale = (float)(1 << (q < n ? q : q - n));
+      const float arg = src * scale;
+      v = q < n ? sinf(arg) : sinf(arg + 1.57079632679489662f);
+    } else {
+      v = 0.0f;
+    }
+    X[i * d.ldE + c] = v;
+  }
+}
+
+// Runs layers 0..2 of the tile. On exit: X holds the encoded input, H0 (if
+// non-null) a copy of h0, H1 = relu(z1), H2p = relu(z2) (pre-FiLM) and H2 the
+// layer-3 input (== H2p without embedding). Fs is read by layer 0 only.
+// Work items are (column tile, group of RTI row tiles): RTI = TM/16 keeps a
+// column strip's B operands for all rows of the tile (few waves), RTI = 1
+// spreads a layer over more waves.
+template <int TM, int NW, bool BF, int RTI>
+__device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
+                               const float* Fs, float* X, float* H0, float* H1, float* H2p,
+                               float* H2, const float* Ds, const int* Cs) {
+  constexpr int RG = TM / 16 / RTI;  // row groups
+  // B prefetch depth (K per pass): the whole strip with few waves, 64 when the
+  // workgroup's waves share the register file (16 waves)
+  constexpr int K0 = RTI == 1 ? 48 : 192, K1 = RTI == 1 ? 64 : 160, K2 = RTI == 1 ? 64 : kU;
+  const int wave = threadIdx.x >> 6;
+  const int lane = lane_id();
+  // layer 0: (TM x F) @ (F x S) -> X[:, 0:S] (2 column tiles of 16)
+  for (int it = wave; it < 2 * RG; it += NW) {
+    const int ct = it / RG, r0 = 16 * RTI * (it % RG);
+    f32x4 acc[RTI];
+    zero_acc(acc);
+    mma_strip<RTI, K0>(Fs + r0 * d.ldF, d.ldF, op.k0(), op.l0(), 16 * ct, acc);
+#pragma unroll
+    for (int rt = 0; rt < RTI; ++rt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e, col = 16 * ct + (lane & 15);
+        if (col < d.S) {
+          X[row * d.ldE + col] = acc[rt][e];
+          if (H0) H0[row * d.ldS + col] = acc[rt][e];
+        }
+      }
+  }
+  __syncthreads();
+  encode_tile<TM>(d, X, Ds);
+  __syncthreads();
+  // layer 1: (TM x EP) @ (EP x U) + b1, relu -> H1
+  for (int it = wave; it < (kU / 16) * RG; it += NW) {
+    const int ct = it / RG, r0 = 16 * RTI * (it % RG);
+    f32x4 acc[RTI];
+    zero_acc(acc);
+    mma_strip<RTI, K1>(X + r0 * d.ldE, d.ldE, d.EP, op.l1(), 16 * ct, acc);
+    const int col = 16 * ct + (lane & 15);
+    const float bias = w.b1[col];
+#pragma unroll
+    for (int rt = 0; rt < RTI; ++rt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+        H1[row * d.ldU + col] = fmaxf(acc[rt][e] + bias, 0.0f);
+      }
+  }
+  __syncthreads();
+  // layer 2: (TM x U) @ (U x U) + b2, relu (+ FiLM) -> H2p / H2
+  for (int it = wave; it < (kU / 16) * RG; it += NW) {
+    const int ct = it / RG, r0 = 16 * RTI * (it % RG);
+    f32x4 acc[RTI];
+    zero_acc(acc);
+    mma_strip<RTI, K2>(H1 + r0 * d.ldU, d.ldU, kU, op.l2(), 16 * ct, acc);
+    const int col = 16 * ct + (lane & 15);
+    const float bias = w.b2[col];
+#pragma unroll
+    for (int rt = 0; rt < RTI; ++rt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+        const float h = fmaxf(acc[rt][e] + bias, 0.0f);
+        float out = h;
+        if (d.ncam > 0 && col >= kU / 2) {
+          const float* emb = w.embed + (size_t)Cs[row] * kU;
+          out = emb[col - kU / 2] * h + emb[col];
+        }
+        if (H2p != H2) H2p[row * d.ldU + col] = h;
+        H2[row * d.ldU + col] = out;
+      }
+  }
+  __syncthreads();
+}
+
+// layer 3 + sigmoid: R3[row][c] for c < 3 (other columns untouched).
+template <int TM, int NW, bool BF>
+__device__ void forward_out(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
+                            const float* H2, float* R3) {
+  constexpr int RT = TM / 16;
+  const int wave = threadIdx.x >> 6;
+  const int lane = lane_id();
+  for (int rt = wave; rt < RT; rt += NW) {
+    f32x4 acc[1];
+    zero_acc(acc);
+    mma_strip<1, kU>(H2 + 16 * rt * d.ldU, d.ldU, kU, op.l3(), 0, acc);
+    const int col = lane & 15;
+    if (col < 3) {
+      const float bias = w.b3[col];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 16 * rt + 4 * (lane >> 4) + e;
+        const float z = acc[0][e] + bias;
+        R3[row * d.ld3 + col] = 1.0f / (1.0f + expf(-z));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---- forward kernel -----------------------------------------------------------
+template <int TM>
+inline size_t fwd_lds_floats(const MlpDims& d) {
+  return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + d.ld3 + 4);
+}
+
+template <int TM, int NW, bool BF, bool STD>
+__global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpPack pk,
+                                                     const float* __restrict__ feats,
+                                                     const float* __restrict__ dirs,
+                                                     const uint32_t* __restrict__ cams,
+                                                     int64_t M, int64_t rpr,
+                                                     float* __restrict__ rgb) {
+  const MlpDims d = STD ? std_dims<BF>(d_arg.ncam) : d_arg;
+  extern __shared__ float smem[];
+  // regions: A = Fs, later H1; B = X, later H2; small: Ds, R3, Cs
+  float* regA = smem;
+  float* regB = regA + TM * std::max(d.ldF, d.ldU);
+  float* Ds = regB + TM * std::max(d.ldE, d.ldU);
+  float* R3 = Ds + TM * 4;
+  int* Cs = reinterpret_cast<int*>(R3 + TM * d.ld3);
+  const Operands<BF> op{d, w, pk};
+  const int64_t row0 = (int64_t)blockIdx.x * TM;
+  load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, regA, Ds, Cs);
+  __syncthreads();
+  // layer 1 reads X (regB) and writes H1 over Fs (regA, dead after layer 0);
+  // layer 2 reads H1 and writes H2 over X (dead after layer 1)
+  forward_hidden<TM, NW, BF, TM / 16>(d, w, op, regA, regB, nullptr, regA, regB, regB, Ds, Cs);
+  forward_out<TM, NW, BF>(d, w, op, regB, R3);
+  for (int e = threadIdx.x; e < TM * 3; e += blockDim.x) {
+    const int i = e / 3, c = e % 3;
+    const int64_t row = row0 + i;
+    if (row < M) rgb[row * 3 + c] = R3[i * d.ld3 + c];
+  }
+}
+
+// ---- backward ---------------------------------------------------------------------
+// Packed layout of one workgroup's gradient partial (and of the reduction).
+struct GradPack {
+  int64_t w0, w1, b1, w2, b2, w3, b3, total;
+};
+
+__host__ __device__ constexpr GradPack make_pack(const MlpDims& d) {
+  GradPack p{};
+  int64_t o = 0;
+  p.w0 = o; o += (int64_t)d.F * d.S;
+  p.w1 = o; o += (int64_t)d.E * kU;
+  p.b1 = o; o += kU;
+  p.w2 = o; o += (int64_t)kU * kU;
+  p.b2 = o; o += kU;
+  p.w3 = o; o += (int64_t)kU * 3;
+  p.b3 = o; o += 3;
+  p.total = (o + 63) / 64 * 64;
+  return p;
+}
+
+// Operand tables written by k_pack_weights (element type T: __bf16 or
+// float). One thread per table entry; entries outside the real matrix are zero.
+template <typename T>
+struct PackOut {
+  T *f0, *f1, *f2, *f3, *g0, *g1, *g2, *g3;
+};
+
+struct PackSizes {
+  int64_t f0, f1, f2, f3;      // forward tables
+  int64_t g0, g1, g2, g3;      // input-gradient tables
+};
+
+inline PackSizes pack_sizes(const MlpDims& d) {
+  PackSizes s;
+  s.f0 = (int64_t)32 * d.FP; s.f1 = (int64_t)kU * d.EP; s.f2 = kU * kU; s.f3 = 16 * kU;
+  s.g0 = (int64_t)d.FP * 32; s.g1 = (int64_t)d.EP * kU; s.g2 = kU * kU; s.g3 = kU * 32;
+  return s;
+}
+
+__device__ __forceinline__ float wat(const float* w, int rows, int cols, int r, int c) {
+  return (r < rows && c < cols) ? w[r * cols + c] : 0.0f;
+}
+
+template <typename T>
+__global__ void k_pack_weights(MlpDims d, MlpW w, PackOut<T> o, PackSizes s) {
+  const int64_t total = s.f0 + s.f1 + s.f2 + s.f3 + s.g0 + s.g1 + s.g2 + s.g3;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t f = e;
+    // forward tables T[j][k] = W[k][j]
+    if (f < s.f0) { const int j = f / d.FP, k = f % d.FP; o.f0[f] = (T)wat(w.w0, d.F, d.S, k, j); continue; }
+    f -= s.f0;
+    if (f < s.f1) { const int j = f / d.EP, k = f % d.EP; o.f1[f] = (T)wat(w.w1, d.E, kU, k, j); continue; }
+    f -= s.f1;
+    if (f < s.f2) { const int j = f / kU, k = f % kU; o.f2[f] = (T)wat(w.w2, kU, kU, k, j); continue; }
+    f -= s.f2;
+    if (f < s.f3) { const int j = f / kU, k = f % kU; o.f3[f] = (T)wat(w.w3, kU, 3, k, j); continue; }
+    f -= s.f3;
+    // input-gradient tables T[j][k] = W[j][k] (j = input unit, k = output unit)
+    if (f < s.g0) { const int j = f / 32, k = f % 32; o.g0[f] = (T)wat(w.w0, d.F, d.S, j, k); continue; }
+    f -= s.g0;
+    if (f < s.g1) { const int j = f / kU, k = f % kU; o.g1[f] = (T)wat(w.w1, d.E, kU, j, k); continue; }
+    f -= s.g1;
+    if (f < s.g2) { const int j = f / kU, k = f % kU; o.g2[f] = (T)wat(w.w2, kU, kU, j, k); continue; }
+    f -= s.g2;
+    { const int j = f / 32, k = f % 32; o.g3[f] = (T)wat(w.w3, kU, 3, j, k); }
+  }
+}
+
+// Adds a 16x16 weight-gradient tile (D layout) into the workgroup partial at
+// rows [row0, row0+16) x cols [col0, col0+16) of a (rows x cols) block.
+__device__ __forceinline__ void add_tile(float* __restrict__ part, int64_t off, int rows,
+                                         int cols, int row0, int col0, const f32x4& acc) {
+  const int lane = lane_id();
+  const int col = col0 + (lane & 15);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int row = row0 + 4 * (lane >> 4) + e;
+    if (row < rows && col < cols) part[off + (int64_t)row * cols + col] += acc[e];
+  }
+}
+
+template <int TM>
+inline size_t bwd_lds_floats(const MlpDims& d) {
+  return (size_t)TM * (d.ldF + d.ldE + d.ldS + 2 * d.ldU + (d.ncam > 0 ? d.ldU : 0) + d.ld3 + 4 +
+                       4) +
+         (d.ncam > 0 ? kU + 4 : 0);  // FiLM: running embedding gradient + 2 camera ids
+}
+
+// Weight-gradient tiles (16x16) of the four layers, numbered
+// [dW3 | dW2 | dW1 | dW0]. With NT > 0 wave w accumulates tiles
+// w, w+NW, ..., w+(NT-1)NW in registers over all its row tiles and stores
+// them once at the end; with NT == 0 every row tile read-modify-writes the
+// workgroup partial instead (used when the tiles do not fit NT*NW).
+struct WTiles {
+  int n3, n2, n1, n0, b2, b1, b0, total;
+};
+
+__host__ __device__ inline WTiles wtiles(const MlpDims& d) {
+  WTiles t;
+  t.n3 = kU / 16;
+  t.n2 = (kU / 16) * (kU / 16);
+  t.n1 = (d.EP / 16) * (kU / 16);
+  t.n0 = (d.FP / 16) * 2;
+  t.b2 = t.n3;
+  t.b1 = t.b2 + t.n2;
+  t.b0 = t.b1 + t.n1;
+  t.total = t.b0 + t.n0;
+  return t;
+}
+
+// Stores a 16x16 tile (D layout) into the (rows x cols) block at `off`.
+__device__ __forceinline__ void store_tile(float* __restrict__ part, int64_t off, int rows,
+                                           int cols, int row0, int col0, const f32x4& acc) {
+  const int lane = lane_id();
+  const int col = col0 + (lane & 15);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int row = row0 + 4 * (lane >> 4) + e;
+    if (row < rows && col < cols) part[off + (int64_t)row * cols + col] = acc[e];
+  }
+}
+
+// Weight-gradient phase of one layer: for every tile of the layer (n tiles,
+// tcols column tiles per row of tiles) add X^T Y over the TM rows of the row
+// tile. With NT > 0, wave w owns the layer's tiles w, w+NW, ..., held in the
+// register slots [SLOT0, SLOT0 + CNT); otherwise tiles read-modify-write the
+// workgroup partial.
+template <int TM, int NW, bool BF, int NT, int SLOT0, int CNT>
+__device__ __forceinline__ void wgrad_phase(f32x4 (&wacc)[NT > 0 ? NT : 1], int wave, int n,
+                                            int tcols, const float* Xm, int ldx,
+                                            const float* Ym, int ldy, float* part, int64_t off,
+                                            int rows, int cols) {
+  if constexpr (NT > 0) {
+#pragma unroll
+    for (int si = 0; si < CNT; ++si) {
+      const int t = wave + NW * si;
+      if (t < n) {
+        const int ti = t / tcols, tj = t % tcols;
+        mma_wgrad_acc<BF, TM>(Xm, ldx, 16 * ti, Ym, ldy, 16 * tj, wacc[SLOT0 + si]);
+      }
+      // keep the operand loads of later tiles from being hoisted (they would
+      // all be live at once)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+    for (int t = wave; t < n; t += NW) {
+      const int ti = t / tcols, tj = t % tcols;
+      add_tile(part, off, rows, cols, 16 * ti, 16 * tj,
+               mma_wgrad<BF, TM>(Xm, ldx, 16 * ti, Ym, ldy, 16 * tj));
+    }
+  }
+}
+
+// Register slots of the weight-gradient tiles per wave (kBwdNW waves): dW3 8
+// tiles, dW2 64, dW1 up to 80 (encoded width <= 160), dW0 up to 20
+// (features <= 160).
+constexpr int kBwdNW = 8;
+constexpr int kCnt3 = (8 + kBwdNW - 1) / kBwdNW, kSlot3 = 0;
+constexpr int kCnt2 = (64 + kBwdNW - 1) / kBwdNW, kSlot2 = kSlot3 + kCnt3;
+constexpr int kCnt1 = (80 + kBwdNW - 1) / kBwdNW, kSlot1 = kSlot2 + kCnt2;
+constexpr int kCnt0 = (20 + kBwdNW - 1) / kBwdNW, kSlot0 = kSlot1 + kCnt1;
+constexpr int kSlots = kSlot0 + kCnt0;
+
+template <int TM, int NW, bool BF, int NT, bool STD, bool FILM>
+__global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpPack pk, GradPack gp_arg,
+                                                     const float* __restrict__ feats,
+                                                     const float* __restrict__ dirs,
+                                                     const uint32_t* __restrict__ cams,
+                                                     const float* __restrict__ grgb, int64_t M,
+                                                     int64_t rpr, float* __restrict__ gfeat,
+                                                     float* __restrict__ partials,
+                                                     float* __restrict__ gembed) {
+  constexpr int RT = TM / 16;
+  static_assert(TM <= 64, "the FiLM camera check reads one tile row per lane");
+  // row tiles per work item of the strip phases: all of them (the B strip is
+  // loaded once per column tile) unless the workgroup has waves to spare
+  constexpr int BRTI = NW >= 16 ? 1 : RT;
+  constexpr int RG = RT / BRTI;
+  // the specialised shape folds the camera count to 0 without an embedding
+  const MlpDims d = STD ? (FILM ? std_dims<BF>(d_arg.ncam) : std_dims<BF>()) : d_arg;
+  const GradPack gp = STD ? make_pack(std_dims<BF>()) : gp_arg;
+  extern __shared__ float smem[];
+  float* Fs = smem;                         // TM x ldF
+  float* X = Fs + TM * d.ldF;               // TM x ldE   (later dX)
+  float* H0 = X + TM * d.ldE;               // TM x ldS   (later dH0)
+  float* H1 = H0 + TM * d.ldS;              // TM x ldU   (later dZ1)
+  float* H2 = H1 + TM * d.ldU;              // TM x ldU   layer-3 input (later dZ2)
+  float* after_h2 = H2 + TM * d.ldU;
+  float* H2p = d.ncam > 0 ? after_h2 : H2;  // pre-FiLM relu(z2) (own region with FiLM)
+  float* Z3 = d.ncam > 0 ? after_h2 + TM * d.ldU : after_h2;  // TM x ld3: rgb, then dZ3
+  float* Ds = Z3 + TM * d.ld3;
+  int* Cs = reinterpret_cast<int*>(Ds + TM * 4);
+  // FiLM embedding gradient: tiles whose rows share one camera add their
+  // column sums into Eacc (g * h2p for the scale half, g for the shift half);
+  // Eacc goes to gembed with one atomic per column when the camera changes
+  // and at the end. Tiles with several cameras add per row directly.
+  float* Eacc = reinterpret_cast<float*>(Cs + 4 * TM);
+  int* Run = reinterpret_cast<int*>(Eacc + kU);  // [0] tile camera or -1, [1] camera of Eacc or -1
+  const Operands<BF> op{d, w, pk};
+
+  const int wave = threadIdx.x >> 6;
+  const int lane = lane_id();
+  float* part = partials + (int64_t)blockIdx.x * gp.total;
+  if (FILM && d.ncam > 0) {
+    for (int c = threadIdx.x; c < kU; c += blockDim.x) Eacc[c] = 0.0f;
+    if (threadIdx.x == 0) Run[1] = -1;
+  }
+  const WTiles wt = wtiles(d);
+  f32x4 wacc[NT > 0 ? NT : 1];
+#pragma unroll
+  for (int i = 0; i < (NT > 0 ? NT : 1); ++i) wacc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float db1 = 0.f, db2 = 0.f, db3 = 0.f;  // bias gradients of column threadIdx.x
+  if (NT == 0)
+    for (int64_t e = threadIdx.x; e < gp.total; e += blockDim.x) part[e] = 0.0f;
+
+  const int64_t ntiles = (M + TM - 1) / TM;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t row0 = tile * TM;
+    __syncthreads();
+    load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
+    __syncthreads();
+    if (FILM && d.ncam > 0 && wave == 0) {
+      // read before the next use: forward_hidden synchronises the workgroup
+      const int nvalid = (int)min((int64_t)TM, M - row0);
+      const int c0 = Cs[0];
+      const int tc = __all(lane >= nvalid || Cs[lane] == c0) ? c0 : -1;  // TM <= 64
+      const int rc = Run[1];
+      if (tc >= 0 && rc >= 0 && rc != tc) {
+        for (int c = lane; c < kU; c += 64) {
+          unsafeAtomicAdd(gembed + (size_t)rc * kU + c, Eacc[c]);
+          Eacc[c] = 0.0f;
+        }
+      }
+      if (lane == 0) {
+        Run[0] = tc;
+        if (tc >= 0) Run[1] = tc;
+      }
+    }
+    forward_hidden<TM, NW, BF, BRTI>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
+    forward_out<TM, NW, BF>(d, w, op, H2, Z3);
+    // dZ3 = grgb * s * (1 - s), zero outside the 3 colours and the valid rows
+    for (int e = threadIdx.x; e < TM * d.Z3W; e += blockDim.x) {
+      const int i = e / d.Z3W, c = e % d.Z3W;
+      const int64_t row = row0 + i;
+      float v = 0.0f;
+      if (c < 3 && row < M) {
+        const float s = Z3[i * d.ld3 + c];
+        v = grgb[row * 3 + c] * s * (1.0f - s);
+      }
+      Z3[i * d.ld3 + c] = v;
+    }
+    __syncthreads();
+    // dW3 (U x 3) += H2^T dZ3 ; db3
+    wgrad_phase<TM, NW, BF, NT, kSlot3, kCnt3>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3, d.ld3, part,
+                                               gp.w3, kU, 3);
+    if (threadIdx.x < 3)
+      for (int i = 0; i < TM; ++i) db3 += Z3[i * d.ld3 + threadIdx.x];
+    __syncthreads();
+    // Strip phases below run one work item per (column tile, row group).
+    // dH2 = dZ3 @ W3^T; FiLM backward; dZ2 = dH2 * (h2 > 0) written over H2
+    for (int it = wave; it < (kU / 16) * RG; it += NW) {
+      const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
+      f32x4 acc[BRTI];
+      zero_acc(acc);
+      mma_strip<BRTI, 32>(Z3 + r0 * d.ld3, d.ld3, op.k3(), op.g3(), 16 * ct, acc);
+      const int col = 16 * ct + (lane & 15);
+      if constexpr (FILM) {
+        const bool film = d.ncam > 0 && 16 * ct >= kU / 2;  // wave-uniform
+        const int tcam = d.ncam > 0 ? Run[0] : -1;
+        float sh = 0.0f, sg = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4 * BRTI; ++q) {
+          const int rt = q / 4, e = q % 4;
+          const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+          float g = acc[rt][e];
+          const float hp = H2p[row * d.ldU + col];
+          if (film) {
+            if (row0 + row < M) {
+              if (tcam >= 0) {
+                sh += g * hp;
+                sg += g;
+              } else {
+                float* gemb = gembed + (size_t)Cs[row] * kU;
+                unsafeAtomicAdd(gemb + col - kU / 2, g * hp);
+                unsafeAtomicAdd(gemb + col, g);
+              }
+            }
+            g = g * w.embed[(size_t)Cs[row] * kU + col - kU / 2];
+          }
+          H2[row * d.ldU + col] = hp > 0.0f ? g : 0.0f;
+        }
+        if (film && tcam >= 0) {
+          sh += __shfl_xor(sh, 16, 64);
+          sh += __shfl_xor(sh, 32, 64);
+          sg += __shfl_xor(sg, 16, 64);
+          sg += __shfl_xor(sg, 32, 64);
+          if (lane < 16) {
+            atomicAdd(Eacc + col - kU / 2, sh);  // LDS: row groups of one column may share it
+            atomicAdd(Eacc + col, sg);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4 * BRTI; ++q) {
+          const int rt = q / 4, e = q % 4;
+          const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+          const float hp = H2p[row * d.ldU + col];
+          H2[row * d.ldU + col] = hp > 0.0f ? acc[rt][e] : 0.0f;
+        }
+      }
+    }
+    __syncthreads();
+    // dW2 (U x U) += H1^T dZ2 ; db2
+    wgrad_phase<TM, NW, BF, NT, kSlot2, kCnt2>(wacc, wave, wt.n2, kU / 16, H1, d.ldU, H2,
+                                               d.ldU, part, gp.w2, kU, kU);
+    if (threadIdx.x < kU)
+      for (int i = 0; i < TM; ++i) db2 += H2[i * d.ldU + threadIdx.x];
+    __syncthreads();
+    // dH1 = dZ2 @ W2^T ; dZ1 = dH1 * (h1 > 0) written over H1 (dW2 is done)
+    for (int it = wave; it < (kU / 16) * RG; it += NW) {
+      const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
+      f32x4 acc[BRTI];
+      zero_acc(acc);
+      mma_strip<BRTI, (BRTI == 1 ? 64 : kU)>(H2 + r0 * d.ldU, d.ldU, kU, op.g2(), 16 * ct, acc);
+      const int col = 16 * ct + (lane & 15);
+#pragma unroll
+      for (int q = 0; q < 4 * BRTI; ++q) {
+        const int rt = q / 4, e = q % 4;
+        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+        const float h = H1[row * d.ldU + col];
+        H1[row * d.ldU + col] = h > 0.0f ? acc[rt][e] : 0.0f;
+      }
+    }
+    __syncthreads();
+    // dW1 (E x U) += X^T dZ1 ; db1
+    wgrad_phase<TM, NW, BF, NT, kSlot1, kCnt1>(wacc, wave, wt.n1, kU / 16, X, d.ldE, H1,
+                                               d.ldU, part, gp.w1, d.E, kU);
+    if (threadIdx.x < kU)
+      for (int i = 0; i < TM; ++i) db1 += H1[i * d.ldU + threadIdx.x];
+    __syncthreads();
+    // dX = dZ1 @ W1^T for the columns that feed h0: [0, S) and the FE(h0) block
+    {
+      const int fe0 = d.S + 3, fe1 = fe0 + 2 * d.nf * d.S;
+      const int nct = d.EP / 16;
+      for (int it = wave; it < nct * RG; it += NW) {
+        const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
+        const int c0 = 16 * ct;
+        const bool needed = (c0 < d.S) || (c0 + 16 > fe0 && c0 < fe1);
+        if (!needed) continue;
+        f32x4 acc[BRTI];
+        zero_acc(acc);
+        mma_strip<BRTI, (BRTI == 1 ? 64 : kU)>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
+        const int col = c0 + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 4 * BRTI; ++q) {
+          const int rt = q / 4, e = q % 4;
+          const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+          X[row * d.ldE + col] = acc[rt][e];  // dX (X is dead after dW1)
+        }
+      }
+    }
+    __syncthreads();
+    // dH0[j] = dX[j] + sum_q dFE terms (networks.py:13-35); stored in H0 (zero padded)
+    {
+      const int fe0 = d.S + 3;
+      for (int e = threadIdx.x; e < TM * kSP; e += blockDim.x) {
+        const int i = e / kSP, j = e % kSP;
+        float g = 0.0f;
+        if (j < d.S) {
+          const float h = H0[i * d.ldS + j];
+          g = X[i * d.ldE + j];
+          for (int q = 0; q < d.nf; ++q) {
+            const float s = (float)(1 << q);
+            const float a = h * s;
+            g += X[i * d.ldE + fe0 + j * 2 * d.nf + q] * s * cosf(a);
+            g += X[i * d.ldE + fe0 + j * 2 * d.nf + d.nf + q] * s *
+                 cosf(a + 1.57079632679489662f);
+          }
+        }
+        H0[i * d.ldS + j] = g;
+      }
+    }
+    __syncthreads();
+    // dW0 (F x S) += F^T dH0
+    wgrad_phase<TM, NW, BF, NT, kSlot0, kCnt0>(wacc, wave, wt.n0, 2, Fs, d.ldF, H0, d.ldS,
+                                               part, gp.w0, d.F, d.S);
+    // dF = dH0 @ W0^T -> global
+    if (gfeat) {
+      for (int it = wave; it < (d.FP / 16) * RG; it += NW) {
+        const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
+        f32x4 acc[BRTI];
+        zero_acc(acc);
+        mma_strip<BRTI, kSP>(H0 + r0 * d.ldS, d.ldS, kSP, op.g0(), 16 * ct, acc);
+        const int col = 16 * ct + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 4 * BRTI; ++q) {
+          const int rt = q / 4, e = q % 4;
+          const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+          const int64_t grow = row0 + row;
+          if (grow < M && col < d.F) gfeat[grow * d.F + col] = acc[rt][e];
+        }
+      }
+    }
+  }
+  if (FILM && d.ncam > 0) {
+    __syncthreads();
+    const int rc = Run[1];
+    if (rc >= 0)
+      for (int c = threadIdx.x; c < kU; c += blockDim.x)
+        unsafeAtomicAdd(gembed + (size_t)rc * kU + c, Eacc[c]);
+  }
+  // the workgroup's gradient sums -> its partial
+  if constexpr (NT > 0) {
+#pragma unroll
+    for (int si = 0; si < kSlots; ++si) {
+      if (si < kSlot2) {
+        const int t = wave + NW * (si - kSlot3);
+        if (t < wt.n3) store_tile(part, gp.w3, kU, 3, 16 * t, 0, wacc[si]);
+      } else if (si < kSlot1) {
+        const int t = wave + NW * (si - kSlot2);
+        if (t < wt.n2)
+          store_tile(part, gp.w2, kU, kU, 16 * (t / (kU / 16)), 16 * (t % (kU / 16)), wacc[si]);
+      } else if (si < kSlot0) {
+        const int t = wave + NW * (si - kSlot1);
+        if (t < wt.n1)
+          store_tile(part, gp.w1, d.E, kU, 16 * (t / (kU / 16)), 16 * (t % (kU / 16)), wacc[si]);
+      } else {
+        const int t = wave + NW * (si - kSlot0);
+        if (t < wt.n0) store_tile(part, gp.w0, d.F, d.S, 16 * (t / 2), 16 * (t % 2), wacc[si]);
+      }
+    }
+    if (threadIdx.x < kU) {
+      part[gp.b1 + threadIdx.x] = db1;
+      part[gp.b2 + threadIdx.x] = db2;
+    }
+    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] = db3;
+  } else {
+    __syncthreads();
+    if (threadIdx.x < kU) {
+      part[gp.b1 + threadIdx.x] += db1;
+      part[gp.b2 + threadIdx.x] += db2;
+    }
+    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] += db3;
+  }
+}
+
+constexpr int kFwdTM = 32, kFwdNW = 4;
+constexpr int kBwdTM = 32;
+constexpr int kBwdNT = kSlots;  // register-resident weight-gradient tiles per wave
+
+// Dynamic LDS above 64 KB needs the per-kernel opt-in.
+template <class K>
+inline void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipGetLastError();
+  }
+}
+
+// Backward launches of the standard shape without (tfh_mlp_bwd_std.hip) and
+// with (tfh_mlp_film.hip) a camera embedding.
+void launch_mlp_bwd_std(bool bf, unsigned nblk, size_t lds, hipStream_t st, const MlpDims& d,
+                        const MlpW& w, const MlpPack& pk, const GradPack& gp,
+                        const float* feats, const float* dirs, const uint32_t* cams,
+                        const float* grgb, int64_t M, int64_t rpr, float* gfeat,
+                        float* partials, float* gembed);
+void launch_mlp_bwd_std_film(bool bf, unsigned nblk, size_t lds, hipStream_t st,
+                             const MlpDims& d, const MlpW& w, const MlpPack& pk,
+                             const GradPack& gp, const float* feats, const float* dirs,
+                             const uint32_t* cams, const float* grgb, int64_t M, int64_t rpr,
+                             float* gfeat, float* partials, float* gembed);
+
+}  // namespace tfh
