@@ -283,7 +283,7 @@ def main():
     ap.add_argument("--appearance-samples", type=int, default=512)
     ap.add_argument("--grid", type=int, default=300)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
-    ap.add_argument("--cpu-sample-rays", type=int, default=512)
+    ap.add_argument("--cpu-sample-rays", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stages", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
