@@ -180,31 +180,33 @@ __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
 
 // Fills X[:, S:EP] = [d, FE(h0, nf), FE(d, nv), 0...] from h0 = X[:, 0:S]
 // (networks.py:13-35, 68-76): FE(v, n)[j*2n + q] = sin(2^q v_j) for q < n and
-// sin(2^(q-n) v_j + pi/2) otherwise.
+// sin(2^(q-n) v_j + pi/2) = cos(2^(q-n) v_j) otherwise; one sincosf per
+// (row, value, frequency) writes both.
 template <int TM>
 __device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
-  const int ncol = d.EP - d.S;
   const int fe0 = d.S + 3;
   const int fe1 = fe0 + 2 * d.nf * d.S;
   const int fe2 = fe1 + 2 * d.nv * 3;
-  for (int e = threadIdx.x; e < TM * ncol; e += blockDim.x) {
-    const int i = e / ncol, c = d.S + e % ncol;
-    float v;
-    if (c < fe0) {
-      v = Ds[i * 3 + (c - d.S)];
-    } else if (c < fe2) {
-      const bool feat = c < fe1;
-      const int t = feat ? c - fe0 : c - fe1;
-      const int n = feat ? d.nf : d.nv;
-      const int j = t / (2 * n), q = t % (2 * n);
-      const float src = feat ? X[i * d.ldE + j] : Ds[i * 3 + j];
-      const float scale = (float)(1 << (q < n ? q : q - n));
-      const float arg = src * scale;
-      v = q < n ? sinf(arg) : sinf(arg + 1.57079632679489662f);
-    } else {
-      v = 0.0f;
-    }
-    X[i * d.ldE + c] = v;
+  // the view direction and the zero padding past the Fourier features
+  const int nplain = 3 + (d.EP - fe2);
+  for (int e = threadIdx.x; e < TM * nplain; e += blockDim.x) {
+    const int i = e / nplain, t = e % nplain;
+    const int c = t < 3 ? d.S + t : fe2 + (t - 3);
+    X[i * d.ldE + c] = t < 3 ? Ds[i * 3 + t] : 0.0f;
+  }
+  const int nfe = d.S * d.nf + 3 * d.nv;  // (value, frequency) pairs per row
+  for (int e = threadIdx.x; e < TM * nfe; e += blockDim.x) {
+    const int i = e / nfe, t = e % nfe;
+    const bool feat = t < d.S * d.nf;
+    const int u = feat ? t : t - d.S * d.nf;
+    const int n = feat ? d.nf : d.nv;
+    const int j = u / n, q = u % n;
+    const float src = feat ? X[i * d.ldE + j] : Ds[i * 3 + j];
+    float sv, cv;
+    sincosf(src * (float)(1 << q), &sv, &cv);
+    const int c = (feat ? fe0 : fe1) + j * 2 * n + q;
+    X[i * d.ldE + c] = sv;
+    X[i * d.ldE + c + n] = cv;
   }
 }
 
@@ -224,22 +226,21 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
   constexpr int K0 = RTI == 1 ? 48 : 192, K1 = RTI == 1 ? 64 : 160, K2 = RTI == 1 ? 64 : kU;
   const int wave = threadIdx.x >> 6;
   const int lane = lane_id();
-  // layer 0: (TM x F) @ (F x S) -> X[:, 0:S] (2 column tiles of 16)
-  for (int it = wave; it < 2 * RG; it += NW) {
-    const int ct = it / RG, r0 = 16 * RTI * (it % RG);
-    f32x4 acc[RTI];
+  // layer 0: (TM x F) @ (F x S) -> X[:, 0:S]: 2 column tiles of 16, one row
+  // tile per item so that 2 * TM/16 waves share it
+  for (int it = wave; it < 2 * (TM / 16); it += NW) {
+    const int ct = it / (TM / 16), r0 = 16 * (it % (TM / 16));
+    f32x4 acc[1];
     zero_acc(acc);
-    mma_strip<RTI, K0>(Fs + r0 * d.ldF, d.ldF, op.k0(), op.l0(), 16 * ct, acc);
+    mma_strip<1, K0>(Fs + r0 * d.ldF, d.ldF, op.k0(), op.l0(), 16 * ct, acc);
 #pragma unroll
-    for (int rt = 0; rt < RTI; ++rt)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = r0 + 16 * rt + 4 * (lane >> 4) + e, col = 16 * ct + (lane & 15);
-        if (col < d.S) {
-          X[row * d.ldE + col] = acc[rt][e];
-          if (H0) H0[row * d.ldS + col] = acc[rt][e];
-        }
+    for (int e = 0; e < 4; ++e) {
+      const int row = r0 + 4 * (lane >> 4) + e, col = 16 * ct + (lane & 15);
+      if (col < d.S) {
+        X[row * d.ldE + col] = acc[0][e];
+        if (H0) H0[row * d.ldS + col] = acc[0][e];
       }
+    }
   }
   __syncthreads();
   encode_tile<TM>(d, X, Ds);
@@ -689,24 +690,22 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     if (threadIdx.x < kU)
       for (int i = 0; i < TM; ++i) db1 += H1[i * d.ldU + threadIdx.x];
     __syncthreads();
-    // dX = dZ1 @ W1^T for the columns that feed h0: [0, S) and the FE(h0) block
+    // dX = dZ1 @ W1^T for the columns that feed h0: [0, S) and the FE(h0)
+    // block, i.e. the column tiles below fe1 (the 3 direction columns between
+    // them are inside a tile); one row tile per item balances the waves
     {
-      const int fe0 = d.S + 3, fe1 = fe0 + 2 * d.nf * d.S;
-      const int nct = d.EP / 16;
-      for (int it = wave; it < nct * RG; it += NW) {
-        const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
-        const int c0 = 16 * ct;
-        const bool needed = (c0 < d.S) || (c0 + 16 > fe0 && c0 < fe1);
-        if (!needed) continue;
-        f32x4 acc[BRTI];
+      const int fe1 = d.S + 3 + 2 * d.nf * d.S;
+      const int ndx = (fe1 + 15) / 16;
+      for (int it = wave; it < ndx * RT; it += NW) {
+        const int c0 = 16 * (it / RT), r0 = 16 * (it % RT);
+        f32x4 acc[1];
         zero_acc(acc);
-        mma_strip<BRTI, (BRTI == 1 ? 64 : kU)>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
+        mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
         const int col = c0 + (lane & 15);
 #pragma unroll
-        for (int q = 0; q < 4 * BRTI; ++q) {
-          const int rt = q / 4, e = q % 4;
-          const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
-          X[row * d.ldE + col] = acc[rt][e];  // dX (X is dead after dW1)
+        for (int e = 0; e < 4; ++e) {
+          const int row = r0 + 4 * (lane >> 4) + e;
+          X[row * d.ldE + col] = acc[0][e];  // dX (X is dead after dW1)
         }
       }
     }
@@ -722,10 +721,11 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
           g = X[i * d.ldE + j];
           for (int q = 0; q < d.nf; ++q) {
             const float s = (float)(1 << q);
-            const float a = h * s;
-            g += X[i * d.ldE + fe0 + j * 2 * d.nf + q] * s * cosf(a);
-            g += X[i * d.ldE + fe0 + j * 2 * d.nf + d.nf + q] * s *
-                 cosf(a + 1.57079632679489662f);
+            float sv, cv;
+            sincosf(h * s, &sv, &cv);
+            // d/dh sin(s h) = s cos(s h), d/dh cos(s h) = -s sin(s h)
+            g += X[i * d.ldE + fe0 + j * 2 * d.nf + q] * s * cv;
+            g -= X[i * d.ldE + fe0 + j * 2 * d.nf + d.nf + q] * s * sv;
           }
         }
         H0[i * d.ldS + j] = g;
@@ -737,18 +737,17 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
                                                part, gp.w0, d.F, d.S);
     // dF = dH0 @ W0^T -> global
     if (gfeat) {
-      for (int it = wave; it < (d.FP / 16) * RG; it += NW) {
-        const int ct = it / RG, r0 = 16 * BRTI * (it % RG);
-        f32x4 acc[BRTI];
+      for (int it = wave; it < (d.FP / 16) * RT; it += NW) {
+        const int ct = it / RT, r0 = 16 * (it % RT);
+        f32x4 acc[1];
         zero_acc(acc);
-        mma_strip<BRTI, kSP>(H0 + r0 * d.ldS, d.ldS, kSP, op.g0(), 16 * ct, acc);
+        mma_strip<1, kSP>(H0 + r0 * d.ldS, d.ldS, kSP, op.g0(), 16 * ct, acc);
         const int col = 16 * ct + (lane & 15);
 #pragma unroll
-        for (int q = 0; q < 4 * BRTI; ++q) {
-          const int rt = q / 4, e = q % 4;
-          const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
+        for (int e = 0; e < 4; ++e) {
+          const int row = r0 + 4 * (lane >> 4) + e;
           const int64_t grow = row0 + row;
-          if (grow < M && col < d.F) gfeat[grow * d.F + col] = acc[rt][e];
+          if (grow < M && col < d.F) gfeat[grow * d.F + col] = acc[0][e];
         }
       }
     }
