@@ -794,7 +794,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   }
 }
 
-constexpr int kFwdTM = 32, kFwdNW = 4;
+constexpr int kFwdTM = 32, kFwdNW = 8;
 constexpr int kBwdTM = 32;
 constexpr int kBwdNT = kSlots;  // register-resident weight-gradient tiles per wave
 
