@@ -167,18 +167,18 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   if (!cv.ok)
     return fail(TFH_ERR_WORKSPACE, "mlp bwd: workspace of %zu bytes, need %zu", ws_bytes,
                 mlp_bwd_workspace(p, M, precision));
-  const size_t lds = bwd_lds_floats<kBwdTM>(d) * sizeof(float);
+  // TFH_MLP_WGRAD=partial forces the read-modify-write accumulation of the
+  // weight gradients (the register-resident one is the default when the
+  // shape has a specialised kernel)
+  static const bool force_partial = [] {
+    const char* v = getenv("TFH_MLP_WGRAD");
+    return v && strcmp(v, "partial") == 0;
+  }();
+  const bool sd = is_std_dims(d, bf) && !force_partial;
+  const size_t lds = bwd_lds_floats<kBwdTM>(d, sd ? kLdsSlots * kBwdNW : 0) * sizeof(float);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp bwd: tile needs %zu bytes of LDS", lds);
   {
     ProfScope ps("mlp_bwd", st);
-    // TFH_MLP_WGRAD=partial forces the read-modify-write accumulation of the
-    // weight gradients (the register-resident one is the default when the
-    // shape has a specialised kernel)
-    static const bool force_partial = [] {
-      const char* v = getenv("TFH_MLP_WGRAD");
-      return v && strcmp(v, "partial") == 0;
-    }();
-    const bool sd = is_std_dims(d, bf) && !force_partial;
 #define TFH_LAUNCH_BWD(BFV, NTV, STDV, FILMV)                                                 \
   do {                                                                                        \
     allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV, FILMV>, lds);                         \

@@ -432,10 +432,11 @@ __device__ __forceinline__ void add_tile(float* __restrict__ part, int64_t off, 
 }
 
 template <int TM>
-inline size_t bwd_lds_floats(const MlpDims& d) {
+inline size_t bwd_lds_floats(const MlpDims& d, int lds_tiles) {
   return (size_t)TM * (d.ldF + d.ldE + d.ldS + 2 * d.ldU + (d.ncam > 0 ? d.ldU : 0) + d.ld3 + 4 +
                        4) +
-         (d.ncam > 0 ? kU + 4 : 0);  // FiLM: running embedding gradient + 2 camera ids
+         (d.ncam > 0 ? kU + 4 : 0) +  // FiLM: running embedding gradient + 2 camera ids
+         (size_t)lds_tiles * 256;     // weight-gradient tiles accumulated in LDS
 }
 
 // Weight-gradient tiles (16x16) of the four layers, numbered
@@ -477,12 +478,27 @@ __device__ __forceinline__ void store_tile(float* __restrict__ part, int64_t off
 // tile. With NT > 0, wave w owns the layer's tiles w, w+NW, ..., held in the
 // register slots [SLOT0, SLOT0 + CNT); otherwise tiles read-modify-write the
 // workgroup partial.
-template <int TM, int NW, bool BF, int NT, int SLOT0, int CNT>
+template <int TM, int NW, bool BF, int NT, int SLOT0, int CNT, bool IN_LDS = false>
 __device__ __forceinline__ void wgrad_phase(f32x4 (&wacc)[NT > 0 ? NT : 1], int wave, int n,
                                             int tcols, const float* Xm, int ldx,
                                             const float* Ym, int ldy, float* part, int64_t off,
-                                            int rows, int cols) {
-  if constexpr (NT > 0) {
+                                            int rows, int cols, float* Wl = nullptr) {
+  if constexpr (NT > 0 && IN_LDS) {
+    // the wave's tiles in LDS slots [SLOT0, SLOT0 + CNT): each lane keeps its
+    // own 4 accumulator values contiguous (one 16-byte read and write)
+#pragma unroll
+    for (int si = 0; si < CNT; ++si) {
+      const int t = wave + NW * si;
+      if (t < n) {
+        const int ti = t / tcols, tj = t % tcols;
+        f32x4* slot = reinterpret_cast<f32x4*>(Wl) + ((SLOT0 + si) * NW + wave) * 64 + lane_id();
+        f32x4 acc = *slot;
+        mma_wgrad_acc<BF, TM>(Xm, ldx, 16 * ti, Ym, ldy, 16 * tj, acc);
+        *slot = acc;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else if constexpr (NT > 0) {
 #pragma unroll
     for (int si = 0; si < CNT; ++si) {
       const int t = wave + NW * si;
@@ -503,15 +519,17 @@ __device__ __forceinline__ void wgrad_phase(f32x4 (&wacc)[NT > 0 ? NT : 1], int 
   }
 }
 
-// Register slots of the weight-gradient tiles per wave (kBwdNW waves): dW3 8
-// tiles, dW2 64, dW1 up to 80 (encoded width <= 160), dW0 up to 20
-// (features <= 160).
+// Weight-gradient tiles per wave (kBwdNW waves) of the standard shape: dW2
+// (64 tiles) and dW1 (up to 80, encoded width <= 160) in register slots, the
+// small dW3 (8 tiles) and dW0 (up to 20, features <= 160) in LDS slots, which
+// keeps the kernel's registers within two waves per SIMD.
 constexpr int kBwdNW = 8;
-constexpr int kCnt3 = (8 + kBwdNW - 1) / kBwdNW, kSlot3 = 0;
-constexpr int kCnt2 = (64 + kBwdNW - 1) / kBwdNW, kSlot2 = kSlot3 + kCnt3;
+constexpr int kCnt2 = (64 + kBwdNW - 1) / kBwdNW, kSlot2 = 0;
 constexpr int kCnt1 = (80 + kBwdNW - 1) / kBwdNW, kSlot1 = kSlot2 + kCnt2;
-constexpr int kCnt0 = (20 + kBwdNW - 1) / kBwdNW, kSlot0 = kSlot1 + kCnt1;
-constexpr int kSlots = kSlot0 + kCnt0;
+constexpr int kSlots = kSlot1 + kCnt1;
+constexpr int kCnt3 = (8 + kBwdNW - 1) / kBwdNW, kLSlot3 = 0;
+constexpr int kCnt0 = (20 + kBwdNW - 1) / kBwdNW, kLSlot0 = kLSlot3 + kCnt3;
+constexpr int kLdsSlots = kLSlot0 + kCnt0;
 
 template <int TM, int NW, bool BF, int NT, bool STD, bool FILM>
 __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpPack pk, GradPack gp_arg,
@@ -548,6 +566,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   // and at the end. Tiles with several cameras add per row directly.
   float* Eacc = reinterpret_cast<float*>(Cs + 4 * TM);
   int* Run = reinterpret_cast<int*>(Eacc + kU);  // [0] tile camera or -1, [1] camera of Eacc or -1
+  // LDS slots of the dW3 / dW0 tiles (NT > 0): after the FiLM state
+  float* Wl = reinterpret_cast<float*>(Cs + 4 * TM) + (d.ncam > 0 ? kU + 4 : 0);
   const Operands<BF> op{d, w, pk};
 
   const int wave = threadIdx.x >> 6;
@@ -564,6 +584,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   float db1 = 0.f, db2 = 0.f, db3 = 0.f;  // bias gradients of column threadIdx.x
   if (NT == 0)
     for (int64_t e = threadIdx.x; e < gp.total; e += blockDim.x) part[e] = 0.0f;
+  else
+    for (int e = threadIdx.x; e < kLdsSlots * NW * 256; e += blockDim.x) Wl[e] = 0.0f;
 
   const int64_t ntiles = (M + TM - 1) / TM;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -603,8 +625,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     }
     __syncthreads();
     // dW3 (U x 3) += H2^T dZ3 ; db3
-    wgrad_phase<TM, NW, BF, NT, kSlot3, kCnt3>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3, d.ld3, part,
-                                               gp.w3, kU, 3);
+    wgrad_phase<TM, NW, BF, NT, kLSlot3, kCnt3, true>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3, d.ld3,
+                                                      part, gp.w3, kU, 3, Wl);
     if (threadIdx.x < 3)
       for (int i = 0; i < TM; ++i) db3 += Z3[i * d.ld3 + threadIdx.x];
     __syncthreads();
@@ -733,8 +755,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     }
     __syncthreads();
     // dW0 (F x S) += F^T dH0
-    wgrad_phase<TM, NW, BF, NT, kSlot0, kCnt0>(wacc, wave, wt.n0, 2, Fs, d.ldF, H0, d.ldS,
-                                               part, gp.w0, d.F, d.S);
+    wgrad_phase<TM, NW, BF, NT, kLSlot0, kCnt0, true>(wacc, wave, wt.n0, 2, Fs, d.ldF, H0, d.ldS,
+                                                      part, gp.w0, d.F, d.S, Wl);
     // dF = dH0 @ W0^T -> global
     if (gfeat) {
       for (int it = wave; it < (d.FP / 16) * RT; it += NW) {
@@ -763,20 +785,26 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   if constexpr (NT > 0) {
 #pragma unroll
     for (int si = 0; si < kSlots; ++si) {
-      if (si < kSlot2) {
-        const int t = wave + NW * (si - kSlot3);
-        if (t < wt.n3) store_tile(part, gp.w3, kU, 3, 16 * t, 0, wacc[si]);
-      } else if (si < kSlot1) {
+      if (si < kSlot1) {
         const int t = wave + NW * (si - kSlot2);
         if (t < wt.n2)
           store_tile(part, gp.w2, kU, kU, 16 * (t / (kU / 16)), 16 * (t % (kU / 16)), wacc[si]);
-      } else if (si < kSlot0) {
+      } else {
         const int t = wave + NW * (si - kSlot1);
         if (t < wt.n1)
           store_tile(part, gp.w1, d.E, kU, 16 * (t / (kU / 16)), 16 * (t % (kU / 16)), wacc[si]);
+      }
+    }
+    // the LDS slots are the wave's own: no barrier needed
+    const f32x4* wl = reinterpret_cast<const f32x4*>(Wl);
+    for (int si = 0; si < kLdsSlots; ++si) {
+      const f32x4 v = wl[(si * NW + wave) * 64 + lane];
+      if (si < kLSlot0) {
+        const int t = wave + NW * (si - kLSlot3);
+        if (t < wt.n3) store_tile(part, gp.w3, kU, 3, 16 * t, 0, v);
       } else {
-        const int t = wave + NW * (si - kSlot0);
-        if (t < wt.n0) store_tile(part, gp.w0, d.F, d.S, 16 * (t / 2), 16 * (t % 2), wacc[si]);
+        const int t = wave + NW * (si - kLSlot0);
+        if (t < wt.n0) store_tile(part, gp.w0, d.F, d.S, 16 * (t / 2), 16 * (t % 2), v);
       }
     }
     if (threadIdx.x < kU) {
