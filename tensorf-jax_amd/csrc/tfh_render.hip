@@ -377,6 +377,8 @@ __global__ void __launch_bounds__(64) k_composite_bwd(tfh_vm dvm, BwdIn in, floa
     for (int m = 0; m < 3; ++m) {
       TapAddr a;
       tap_addrs(mt[m], m, G, a);
+      TapPlan<16> plan;
+      tap_plan<16>(a, act, plan);
 #pragma unroll
       for (int q = 0; q < kMaxQ; ++q) {
         if (16 * q < C) {
@@ -384,7 +386,7 @@ __global__ void __launch_bounds__(64) k_composite_bwd(tfh_vm dvm, BwdIn in, floa
           const bool on = act && c < C;
           float lv = 0.f, pv = 0.f;
           if (on) tap_values(dvm.plane, dvm.line, C, c, a, lv, pv);
-          tap_scatter(dgrad.plane, dgrad.line, C, c, a, g * pv, g * lv, on);
+          tap_scatter_plan<16>(dgrad.plane, dgrad.line, C, c, a, plan, g * pv, g * lv, on);
         }
       }
     }

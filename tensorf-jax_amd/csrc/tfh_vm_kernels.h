@@ -100,7 +100,7 @@ inline int32_t launch_vm_gather(const tfh_vm& vm, const Src& src, int64_t M, flo
 
 // VJP: grad (per sample, per channel) -> atomic scatter into the factor
 // gradients. Each wave holds 64/LPS consecutive samples; equal texels among
-// them are merged before the atomic (merged_atomic_add).
+// them, in any tap slot, are merged before the atomic (tap_plan).
 template <class Src, bool ROW_MAJOR, int LPS>
 __global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src src, int64_t M,
                                                                const float* __restrict__ gout,
@@ -129,6 +129,10 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src sr
   for (int m = 0; m < 3; ++m) {
     TapAddr a;
     tap_addrs(mt[m], m, G, a);
+    // cross-slot merge for 16-lane groups (4 samples a wave); 8-lane groups
+    // (narrow fields) merge per slot
+    TapPlan<LPS> plan;
+    if constexpr (LPS == 16) tap_plan<LPS>(a, active, plan);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int c = l + LPS * q;
@@ -139,7 +143,10 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src sr
           tap_values(vm.plane, vm.line, C, c, a, lv, pv);
           g = ROW_MAJOR ? gout[s * 3 * C + m * C + c] : tile[m * C + c][grp];
         }
-        tap_scatter<LPS>(grad.plane, grad.line, C, c, a, g * pv, g * lv, on);
+        if constexpr (LPS == 16)
+          tap_scatter_plan<LPS>(grad.plane, grad.line, C, c, a, plan, g * pv, g * lv, on);
+        else
+          tap_scatter<LPS>(grad.plane, grad.line, C, c, a, g * pv, g * lv, on);
       }
     }
   }
