@@ -27,7 +27,14 @@ struct PropWs {
   float* dfeat = nullptr;
   void* mlp_ws = nullptr;
   size_t mlp_ws_bytes = 0;
+  // rows whose interval-loss gradient is non-zero: their indices and, per
+  // block of kNzBlock rows, counts then offsets (entry nz_nblk: the total)
+  int32_t* nz_idx = nullptr;
+  int32_t* nz_blk = nullptr;
+  int64_t nz_nblk = 0;
 };
+
+constexpr int kNzBlock = 1024;
 
 size_t carve_prop(Carve& cv, const tfh_proposal_config* pc, const tfh_vm* field,
                   const tfh_prop_mlp* mlp, int64_t R, PropWs& ws);

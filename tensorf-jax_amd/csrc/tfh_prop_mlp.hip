@@ -40,12 +40,16 @@ __host__ __device__ inline PropPack prop_pack(int F) {
   return p;
 }
 
+// Row `row` of the tile reads feature row idx[row] (or row when idx is null).
 __device__ __forceinline__ void prop_load(const float* __restrict__ feats, int F, int64_t row0,
-                                          int64_t M, float* Fs) {
+                                          int64_t M, float* Fs,
+                                          const int32_t* __restrict__ idx = nullptr) {
   for (int e = threadIdx.x; e < kPTM * kPF; e += blockDim.x) {
     const int i = e / kPF, c = e % kPF;
     const int64_t row = row0 + i;
-    Fs[i * kPLdF + c] = (row < M && c < F) ? feats[row * F + c] : 0.0f;
+    float v = 0.0f;
+    if (row < M && c < F) v = feats[(idx ? (int64_t)idx[row] : row) * F + c];
+    Fs[i * kPLdF + c] = v;
   }
 }
 
@@ -133,10 +137,16 @@ __device__ __forceinline__ void prop_store_tile(float* part, int off, int rows, 
   }
 }
 
+// Rows: all M, or the first min(M, *count) entries of idx (feature and dz row
+// of each); dfeat is written per processed row.
 __global__ void __launch_bounds__(kPNW * 64, 2) k_prop_mlp_bwd(PropW w, const float* __restrict__ feats,
                                                             const float* __restrict__ dz,
-                                                            int64_t M, float* __restrict__ dfeat,
+                                                            int64_t M_max,
+                                                            const int32_t* __restrict__ idx,
+                                                            const int32_t* __restrict__ count,
+                                                            float* __restrict__ dfeat,
                                                             float* __restrict__ partials) {
+  const int64_t M = count ? min(M_max, (int64_t)*count) : M_max;
   constexpr int RT = kPTM / 16;
   __shared__ float Fs[kPTM * kPLdF];
   __shared__ float H1[kPTM * kPLdH];
@@ -152,11 +162,11 @@ __global__ void __launch_bounds__(kPNW * 64, 2) k_prop_mlp_bwd(PropW w, const fl
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t row0 = tile * kPTM;
     __syncthreads();
-    prop_load(feats, w.F, row0, M, Fs);
+    prop_load(feats, w.F, row0, M, Fs, idx);
     for (int e = threadIdx.x; e < kPTM * 16; e += blockDim.x) {
       const int i = e / 16, c = e % 16;
       const int64_t row = row0 + i;
-      DZ[i * kPLdZ + c] = (c == 0 && row < M) ? dz[row] : 0.0f;
+      DZ[i * kPLdZ + c] = (c == 0 && row < M) ? dz[idx ? (int64_t)idx[row] : row] : 0.0f;
     }
     __syncthreads();
     prop_hidden(w, Fs, H1, H2);
@@ -311,7 +321,8 @@ size_t prop_mlp_bwd_workspace(const tfh_prop_mlp* p, int64_t M) {
 
 int32_t prop_mlp_bwd(const tfh_prop_mlp* p, const float* feats, const float* dz, int64_t M,
                      float* dfeat, const tfh_prop_mlp_grad* g, void* ws, size_t ws_bytes,
-                     hipStream_t st) {
+                     hipStream_t st, const int32_t* idx = nullptr,
+                     const int32_t* count = nullptr) {
   if (M == 0) return TFH_OK;
   const int64_t nblk = prop_bwd_blocks(M);
   const PropPack pp = prop_pack((int)p->feat_dim);
@@ -320,7 +331,7 @@ int32_t prop_mlp_bwd(const tfh_prop_mlp* p, const float* feats, const float* dz,
   {
     ProfScope ps("prop_mlp_bwd", st);
     hipLaunchKernelGGL(k_prop_mlp_bwd, dim3((unsigned)nblk), dim3(kPNW * 64), 0, st, to_pw(p),
-                       feats, dz, M, dfeat, partials);
+                       feats, dz, M, idx, count, dfeat, partials);
   }
   TFH_TRY(check_launch("proposal mlp bwd"));
   hipLaunchKernelGGL(k_prop_reduce, dim3((unsigned)ceil_div(pp.b3 + 1, 256)), dim3(256), 0, st,

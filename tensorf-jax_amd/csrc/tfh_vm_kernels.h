@@ -11,9 +11,12 @@
 
 namespace tfh {
 
+// Each source also gives the number of rows it holds out of the launched M
+// (a device-side count for CompactedPoints).
 struct PlanarPoints {
   const float* p;
   int64_t m;
+  __device__ __forceinline__ int64_t limit(int64_t M) const { return M; }
   __device__ __forceinline__ void get(int64_t s, float& x, float& y, float& z) const {
     x = p[s];
     y = p[m + s];
@@ -25,11 +28,29 @@ struct RaySamples {
   const float* pts;    // (R*N, 3)
   const int32_t* idx;  // (R, k) or null
   int64_t k, n;
+  __device__ __forceinline__ int64_t limit(int64_t M) const { return M; }
   __device__ __forceinline__ void get(int64_t row, float& x, float& y, float& z) const {
     const int64_t r = row / k;
     const int64_t j = row - r * k;
     const int64_t i = idx ? (int64_t)idx[row] : j;
     const float* q = pts + (r * n + i) * 3;
+    x = q[0];
+    y = q[1];
+    z = q[2];
+  }
+};
+
+// Rows listed by a compaction: row r is point idx[r] of pts (N, 3); only the
+// first *count rows exist.
+struct CompactedPoints {
+  const float* pts;
+  const int32_t* idx;
+  const int32_t* count;
+  __device__ __forceinline__ int64_t limit(int64_t M) const {
+    return min(M, (int64_t)*count);
+  }
+  __device__ __forceinline__ void get(int64_t row, float& x, float& y, float& z) const {
+    const float* q = pts + (int64_t)idx[row] * 3;
     x = q[0];
     y = q[1];
     z = q[2];
@@ -110,6 +131,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src sr
   __shared__ float tile[ROW_MAJOR ? 1 : 3 * 64][SPB + 1];
   const int grp = threadIdx.x / LPS, l = threadIdx.x % LPS;
   const int64_t s0 = (int64_t)blockIdx.x * SPB;
+  M = src.limit(M);
+  if (s0 >= M) return;  // the whole block: no wave of it joins a merge
   const int64_t s = s0 + grp;
   const bool active = s < M;
   const int G = (int)vm.grid_dim, C = (int)vm.channels;

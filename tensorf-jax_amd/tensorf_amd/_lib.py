@@ -29,6 +29,11 @@ if not os.path.exists(LIB_PATH):
         f"{LIB_PATH} is missing: build it with `python tensorf-jax_amd/build_lib.py` "
         "(there is no CPU fallback)")
 
+# torch first: the library then binds to the HIP runtime torch has loaded (one
+# runtime per process; loading the library first can leave two, and the second
+# sees no device)
+import torch  # noqa: E402,F401
+
 lib = ctypes.CDLL(LIB_PATH)
 
 c_p = ctypes.c_void_p
