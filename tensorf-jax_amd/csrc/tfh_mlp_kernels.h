@@ -160,10 +160,34 @@ __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
                           const float* __restrict__ dirs, const uint32_t* __restrict__ cams,
                           int64_t row0, int64_t M, int64_t rpr, float* Fs, float* Ds,
                           int* Cs) {
-  for (int e = threadIdx.x; e < TM * d.FP; e += blockDim.x) {
-    const int i = e / d.FP, c = e % d.FP;
-    const int64_t row = row0 + i;
-    Fs[i * d.ldF + c] = (row < M && c < d.F) ? feats[row * d.F + c] : 0.0f;
+  constexpr int kMaxPer = 3;  // float4 loads per thread of the batched path
+  if (d.F % 4 == 0 && (reinterpret_cast<uintptr_t>(feats) & 15) == 0 &&
+      TM * d.FP / 4 <= kMaxPer * (int)blockDim.x) {
+    // 16-byte loads, all issued before the first LDS store (one memory
+    // latency per tile instead of one per loop trip)
+    const int n4 = TM * d.FP / 4, fp4 = d.FP / 4, f4 = d.F / 4;
+    float4 v[kMaxPer];
+#pragma unroll
+    for (int k = 0; k < kMaxPer; ++k) {
+      const int e = threadIdx.x + k * blockDim.x;
+      const int i = e / fp4, c4 = e - i * fp4;
+      const int64_t row = row0 + i;
+      v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (e < n4 && row < M && c4 < f4)
+        v[k] = *reinterpret_cast<const float4*>(feats + row * d.F + 4 * c4);
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxPer; ++k) {
+      const int e = threadIdx.x + k * blockDim.x;
+      const int i = e / fp4, c4 = e - i * fp4;
+      if (e < n4) *reinterpret_cast<float4*>(Fs + i * d.ldF + 4 * c4) = v[k];
+    }
+  } else {
+    for (int e = threadIdx.x; e < TM * d.FP; e += blockDim.x) {
+      const int i = e / d.FP, c = e % d.FP;
+      const int64_t row = row0 + i;
+      Fs[i * d.ldF + c] = (row < M && c < d.F) ? feats[row * d.F + c] : 0.0f;
+    }
   }
   for (int e = threadIdx.x; e < TM * 3; e += blockDim.x) {
     const int i = e / 3, c = e % 3;
