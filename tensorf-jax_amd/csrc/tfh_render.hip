@@ -261,16 +261,21 @@ struct BwdIn {
   int N, k;
 };
 
-__global__ void __launch_bounds__(64) k_composite_bwd(tfh_vm dvm, BwdIn in, float* __restrict__ dc,
-                                                      tfh_vm_grad dgrad) {
+// One workgroup of kCompBwdWaves waves per ray: wave 0 runs the scans, every
+// wave then scatters 4 samples at a time.
+constexpr int kCompBwdWaves = 4;
+
+__global__ void __launch_bounds__(64 * kCompBwdWaves) k_composite_bwd(tfh_vm dvm, BwdIn in,
+                                                                     float* __restrict__ dc,
+                                                                     tfh_vm_grad dgrad) {
   extern __shared__ float sm[];
   const int64_t r = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
   const int N = in.N, k = in.k;
   float* s_c = sm;              // (N, 3) colour of selected samples, 0 otherwise
   float* s_sel = s_c + 3 * N;   // 1 for selected samples
   float* s_ds = s_sel + N;      // dL/ds_i
-  for (int i = lane; i < N; i += 64) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
     s_c[3 * i] = 0.f;
     s_c[3 * i + 1] = 0.f;
     s_c[3 * i + 2] = 0.f;
@@ -279,7 +284,7 @@ __global__ void __launch_bounds__(64) k_composite_bwd(tfh_vm dvm, BwdIn in, floa
   __syncthreads();
   const float e_last = in.ray[4 * r], ssum = in.ray[4 * r + 1], u = in.ray[4 * r + 2];
   const float g0 = in.gout[3 * r], g1 = in.gout[3 * r + 1], g2 = in.gout[3 * r + 2];
-  for (int j = lane; j < k; j += 64) {
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
     const int i = in.idx ? in.idx[r * k + j] : j;
     const float* c = in.rgb + (r * k + j) * 3;
     s_c[3 * i] = c[0];
@@ -293,74 +298,76 @@ __global__ void __launch_bounds__(64) k_composite_bwd(tfh_vm dvm, BwdIn in, floa
     dc[(r * k + j) * 3 + 2] = f * g2;
   }
   __syncthreads();
-  // A = sum_K w c
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-  for (int i = lane; i < N; i += 64) {
-    const float wv = in.w[r * N + i];
-    a0 += wv * s_c[3 * i];
-    a1 += wv * s_c[3 * i + 1];
-    a2 += wv * s_c[3 * i + 2];
-  }
-  a0 = wave_sum(a0);
-  a1 = wave_sum(a1);
-  a2 = wave_sum(a2);
-  const float inv_s = 1.0f / (ssum + 1e-8f);
-  const float gsum = g0 + g1 + g2;
-  const float gA = g0 * a0 + g1 * a1 + g2 * a2;
-  // reverse chunks: exclusive suffix sums of (w c) and (w sel)
-  float c0s = 0.f, c1s = 0.f, c2s = 0.f, css = 0.f;  // carries from later chunks
-  const int nchunks = (N + 63) / 64;
-  for (int cb = nchunks - 1; cb >= 0; --cb) {
-    const int i = cb * 64 + lane;
-    const bool act = i < N;
-    const float wv = act ? in.w[r * N + i] : 0.f;
-    const float sel = act ? s_sel[i] : 0.f;
-    const float cx = act ? s_c[3 * i] : 0.f, cy = act ? s_c[3 * i + 1] : 0.f,
-                cz = act ? s_c[3 * i + 2] : 0.f;
-    float v0 = wv * cx, v1 = wv * cy, v2 = wv * cz, vs = wv * sel;
-    // inclusive suffix within the chunk
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const float t0 = __shfl_down(v0, o, 64), t1 = __shfl_down(v1, o, 64),
-                  t2 = __shfl_down(v2, o, 64), ts = __shfl_down(vs, o, 64);
-      if (lane + o < 64) {
-        v0 += t0;
-        v1 += t1;
-        v2 += t2;
-        vs += ts;
+  if (wave == 0) {
+    // A = sum_K w c
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int i = lane; i < N; i += 64) {
+      const float wv = in.w[r * N + i];
+      a0 += wv * s_c[3 * i];
+      a1 += wv * s_c[3 * i + 1];
+      a2 += wv * s_c[3 * i + 2];
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    const float inv_s = 1.0f / (ssum + 1e-8f);
+    const float gsum = g0 + g1 + g2;
+    const float gA = g0 * a0 + g1 * a1 + g2 * a2;
+    // reverse chunks: exclusive suffix sums of (w c) and (w sel)
+    float c0s = 0.f, c1s = 0.f, c2s = 0.f, css = 0.f;  // carries from later chunks
+    const int nchunks = (N + 63) / 64;
+    for (int cb = nchunks - 1; cb >= 0; --cb) {
+      const int i = cb * 64 + lane;
+      const bool act = i < N;
+      const float wv = act ? in.w[r * N + i] : 0.f;
+      const float sel = act ? s_sel[i] : 0.f;
+      const float cx = act ? s_c[3 * i] : 0.f, cy = act ? s_c[3 * i + 1] : 0.f,
+                  cz = act ? s_c[3 * i + 2] : 0.f;
+      float v0 = wv * cx, v1 = wv * cy, v2 = wv * cz, vs = wv * sel;
+      // inclusive suffix within the chunk
+  #pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const float t0 = __shfl_down(v0, o, 64), t1 = __shfl_down(v1, o, 64),
+                    t2 = __shfl_down(v2, o, 64), ts = __shfl_down(vs, o, 64);
+        if (lane + o < 64) {
+          v0 += t0;
+          v1 += t1;
+          v2 += t2;
+          vs += ts;
+        }
       }
+      v0 += c0s;
+      v1 += c1s;
+      v2 += c2s;
+      vs += css;
+      float x0 = __shfl_down(v0, 1, 64), x1 = __shfl_down(v1, 1, 64), x2 = __shfl_down(v2, 1, 64),
+            xs = __shfl_down(vs, 1, 64);
+      if (lane == 63) {
+        x0 = c0s;
+        x1 = c1s;
+        x2 = c2s;
+        xs = css;
+      }
+      if (act) {
+        const float ei = in.e[r * N + i];
+        const float dA0 = sel * ei * cx - x0, dA1 = sel * ei * cy - x1, dA2 = sel * ei * cz - x2;
+        const float dS = sel * ei - xs;
+        const float dU = e_last * inv_s - u * dS * inv_s;
+        const float dl_da = dU * gA + u * (g0 * dA0 + g1 * dA1 + g2 * dA2) - e_last * gsum;
+        const float sig = in.sigma[r * N + i];
+        s_ds[i] = dl_da * in.delta[r * N + i] * (-expm1f(-sig));
+      }
+      c0s = __shfl(v0, 0, 64);
+      c1s = __shfl(v1, 0, 64);
+      c2s = __shfl(v2, 0, 64);
+      css = __shfl(vs, 0, 64);
     }
-    v0 += c0s;
-    v1 += c1s;
-    v2 += c2s;
-    vs += css;
-    float x0 = __shfl_down(v0, 1, 64), x1 = __shfl_down(v1, 1, 64), x2 = __shfl_down(v2, 1, 64),
-          xs = __shfl_down(vs, 1, 64);
-    if (lane == 63) {
-      x0 = c0s;
-      x1 = c1s;
-      x2 = c2s;
-      xs = css;
-    }
-    if (act) {
-      const float ei = in.e[r * N + i];
-      const float dA0 = sel * ei * cx - x0, dA1 = sel * ei * cy - x1, dA2 = sel * ei * cz - x2;
-      const float dS = sel * ei - xs;
-      const float dU = e_last * inv_s - u * dS * inv_s;
-      const float dl_da = dU * gA + u * (g0 * dA0 + g1 * dA1 + g2 * dA2) - e_last * gsum;
-      const float sig = in.sigma[r * N + i];
-      s_ds[i] = dl_da * in.delta[r * N + i] * (-expm1f(-sig));
-    }
-    c0s = __shfl(v0, 0, 64);
-    c1s = __shfl(v1, 0, 64);
-    c2s = __shfl(v2, 0, 64);
-    css = __shfl(vs, 0, 64);
   }
   __syncthreads();
   // density scatter: every channel of every mode receives dL/ds_i
   const int grp = lane >> 4, l16 = lane & 15;
   const int G = (int)dvm.grid_dim, C = (int)dvm.channels;
-  for (int i0 = 0; i0 < N; i0 += 4) {
+  for (int i0 = 4 * wave; i0 < N; i0 += 4 * kCompBwdWaves) {
     const int i = i0 + grp;
     const bool act = i < N;
     float xh = 0.f, yh = 0.f, zh = 0.f, g = 0.f;
@@ -579,8 +586,8 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
   BwdIn bi{ws.d.pts, ws.d.sigma, ws.d.delta, ws.d.w, ws.d.e, ws.d.ray, ws.rgb, gout, ws.d.idx, N, k};
   {
     ProfScope ps("composite_bwd_density_scatter", st);
-    hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)R), dim3(64), 5 * N * sizeof(float), st,
-                       *density, bi, ws.dc, *dgrad);
+    hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)R), dim3(64 * kCompBwdWaves),
+                       5 * N * sizeof(float), st, *density, bi, ws.dc, *dgrad);
   }
   TFH_TRY(check_launch("render bwd: compositing"));
   TFH_TRY(mlp_bwd_rows(mlp, ws.feats, dirs, cams, M, k, ws.dc, ws.dfeat, mgrad, cfg->mlp_precision,
