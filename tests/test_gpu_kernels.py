@@ -176,9 +176,11 @@ def _camera_layout(rng, ncam, m, layout):
 
 # (cameras, camera layout, rows, appearance channels, Fourier frequencies):
 # 48 channels with 2+2 frequencies run the kernels specialised for that shape,
-# 24 channels with 6+6 (the reference's defaults) the generic ones
+# 24 channels with 6+6 (the reference's defaults) the generic ones, 5
+# channels (15 features, not a multiple of 4) the scalar feature loads
 MLP_CASES = [(None, "random", 1000, 48, 2), (5, "random", 1000, 48, 2),
-             (5, "runs", 20000, 48, 2), (None, "random", 1000, 24, 6), (3, "runs", 5000, 24, 6)]
+             (5, "runs", 20000, 48, 2), (None, "random", 1000, 24, 6), (3, "runs", 5000, 24, 6),
+             (None, "random", 700, 5, 2)]
 
 
 @pytest.mark.parametrize("ncam,layout,m,ca,nf", MLP_CASES)
