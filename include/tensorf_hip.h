@@ -210,6 +210,13 @@ int32_t tfh_feature_mlp_bwd(const tfh_mlp* mlp, const float* features, const flo
 /* Workspace of tfh_render_fwd / tfh_render_bwd / tfh_train_step for R rays. */
 size_t tfh_render_workspace_size(const tfh_render_config* cfg, const tfh_mlp* mlp,
                                  int64_t appearance_channels, int64_t num_rays);
+/* Workspace that lets tfh_train_step keep the feature MLP's hidden
+ * activations (1 KB per shaded sample, standard 48-channel shape without a
+ * camera embedding) between its forward and backward instead of recomputing
+ * layers 1-3; with a tfh_render_workspace_size workspace it recomputes them.
+ * No counterpart in the reference (XLA keeps activations by itself). */
+size_t tfh_train_step_workspace_size(const tfh_render_config* cfg, const tfh_mlp* mlp,
+                                     int64_t appearance_channels, int64_t num_rays);
 /* render_rays (render.py:105-279): rays -> out (R,3) for RGB, (R) for the
  * distance modes. jitter: (N) bbox / (R,N) contraction, NULL = none.
  * base_ts/base_steps: contraction schedule (N) or NULL. gumbel: (N) or NULL.

@@ -111,9 +111,16 @@ size_t mlp_fwd_workspace(const tfh_mlp* p, int32_t precision) {
   return pack_bytes(make_dims(p, bf), bf);
 }
 
+// The standard shape without a camera embedding keeps H1 / H2 for the
+// backward when the caller provides the room (the fused training steps).
+bool mlp_keeps_activations(const tfh_mlp* p, int32_t precision) {
+  const bool bf = precision == TFH_PREC_BF16;
+  return p && !p->embed && is_std_dims(make_dims(p, bf), bf);
+}
+
 int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
                      int64_t M, int64_t rpr, float* rgb, int32_t precision, void* ws,
-                     size_t ws_bytes, hipStream_t st) {
+                     size_t ws_bytes, hipStream_t st, float* act = nullptr) {
   const bool bf = precision == TFH_PREC_BF16;
   MlpDims d = make_dims(p, bf);
   Carve cv(ws, ws_bytes);
@@ -129,7 +136,7 @@ int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
     allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, BFV, STDV>, lds);                                    \
     hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW, BFV, STDV>), dim3((unsigned)blocks),       \
                        dim3(kFwdNW * 64), lds, st, d, to_w(p), pk, feats, dirs, cams, M, rpr,  \
-                       rgb);                                                                 \
+                       rgb, act);                                                                 \
   } while (0)
   if (bf) {
     if (sd) TFH_LAUNCH_FWD(true, true); else TFH_LAUNCH_FWD(true, false);
@@ -155,7 +162,8 @@ size_t mlp_bwd_workspace(const tfh_mlp* p, int64_t M, int32_t precision) {
 int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
                      int64_t M, int64_t rpr, const float* grgb, float* gfeat,
                      const tfh_mlp_grad* g, int32_t precision, void* ws, size_t ws_bytes,
-                     hipStream_t st) {
+                     hipStream_t st, const float* act = nullptr,
+                     const float* rgb = nullptr) {
   const bool bf = precision == TFH_PREC_BF16;
   MlpDims d = make_dims(p, bf);
   GradPack gp = make_pack(d);
@@ -184,10 +192,14 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
     allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV, FILMV>, lds);                         \
     hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV, FILMV>),                    \
                        dim3((unsigned)nblk), dim3(kBwdNW * 64), lds, st, d, to_w(p), pk, gp,  \
-                       feats, dirs, cams, grgb, M, rpr, gfeat, partials, g->embed);           \
+                       feats, dirs, cams, grgb, M, rpr, gfeat, partials, g->embed, nullptr,   \
+                       nullptr);                                                              \
   } while (0)
     const bool film = d.ncam > 0;
-    if (sd && film)
+    if (sd && act && !film)
+      launch_mlp_bwd_std_act(bf, (unsigned)nblk, lds, st, d, to_w(p), pk, gp, feats, dirs, grgb, M,
+                             rpr, gfeat, partials, act, rgb);
+    else if (sd && film)
       launch_mlp_bwd_std_film(bf, (unsigned)nblk, lds, st, d, to_w(p), pk, gp, feats, dirs, cams,
                               grgb, M, rpr, gfeat, partials, g->embed);
     else if (sd)

@@ -1,0 +1,27 @@
+// Backward of the standard FeatureMlp shape without a camera embedding that
+// reads H1, H2 and the output kept by the forward kernel instead of
+// recomputing layers 1-3, in a translation unit of its own (register
+// allocation, see tfh_mlp_kernels.h).
+#include "tfh_mlp_kernels.h"
+
+namespace tfh {
+
+void launch_mlp_bwd_std_act(bool bf, unsigned nblk, size_t lds, hipStream_t st,
+                            const MlpDims& d, const MlpW& w, const MlpPack& pk,
+                            const GradPack& gp, const float* feats, const float* dirs,
+                            const float* grgb, int64_t M, int64_t rpr, float* gfeat,
+                            float* partials, const float* act, const float* rgb) {
+  if (bf) {
+    allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, true, kBwdNT, true, false, true>, lds);
+    hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, true, kBwdNT, true, false, true>), dim3(nblk),
+                       dim3(kBwdNW * 64), lds, st, d, w, pk, gp, feats, dirs, nullptr, grgb, M,
+                       rpr, gfeat, partials, nullptr, act, rgb);
+  } else {
+    allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, false, kBwdNT, true, false, true>, lds);
+    hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, false, kBwdNT, true, false, true>),
+                       dim3(nblk), dim3(kBwdNW * 64), lds, st, d, w, pk, gp, feats, dirs, nullptr,
+                       grgb, M, rpr, gfeat, partials, nullptr, act, rgb);
+  }
+}
+
+}  // namespace tfh

@@ -202,6 +202,39 @@ __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
   }
 }
 
+// Activations the forward kernel kept for the backward: per row 256 floats,
+// relu(z1) then the layer-3 input (no camera embedding); plus the row's
+// sigmoid output from rgb. Loaded into H1, H2 (stride ldU) and Z3 (stride
+// ld3), zero past M; the float4 loads are all issued before the stores.
+template <int TM, int NT>
+__device__ void load_act(const MlpDims& d, const float* __restrict__ act,
+                         const float* __restrict__ rgb, int64_t row0, int64_t M, float* H1,
+                         float* H2, float* Z3) {
+  constexpr int kPer = (TM * 64 + NT - 1) / NT;  // float4 per thread, NT threads
+  float4 v[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = threadIdx.x + k * NT;
+    const int i = e >> 6, c4 = e & 63;
+    v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (e < TM * 64 && row0 + i < M)
+      v[k] = *reinterpret_cast<const float4*>(act + (row0 + i) * 256 + 4 * c4);
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = threadIdx.x + k * NT;
+    const int i = e >> 6, c4 = e & 63;
+    if (e < TM * 64) {
+      float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
+      *reinterpret_cast<float4*>(dst) = v[k];
+    }
+  }
+  for (int e = threadIdx.x; e < TM * 3; e += NT) {
+    const int i = e / 3, c = e % 3;
+    Z3[i * d.ld3 + c] = row0 + i < M ? rgb[(row0 + i) * 3 + c] : 0.0f;
+  }
+}
+
 // Fills X[:, S:EP] = [d, FE(h0, nf), FE(d, nv), 0...] from h0 = X[:, 0:S]
 // (networks.py:13-35, 68-76): FE(v, n)[j*2n + q] = sin(2^q v_j) for q < n and
 // sin(2^(q-n) v_j + pi/2) = cos(2^(q-n) v_j) otherwise; one sincosf per
@@ -240,7 +273,8 @@ __device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
 // Work items are (column tile, group of RTI row tiles): RTI = TM/16 keeps a
 // column strip's B operands for all rows of the tile (few waves), RTI = 1
 // spreads a layer over more waves.
-template <int TM, int NW, bool BF, int RTI>
+// L12 == false stops after the encoding (H1 and H2 come from elsewhere).
+template <int TM, int NW, bool BF, int RTI, bool L12 = true>
 __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
                                const float* Fs, float* X, float* H0, float* H1, float* H2p,
                                float* H2, const float* Ds, const int* Cs) {
@@ -269,6 +303,7 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
   __syncthreads();
   encode_tile<TM>(d, X, Ds);
   __syncthreads();
+  if constexpr (!L12) return;
   // layer 1: (TM x EP) @ (EP x U) + b1, relu -> H1
   for (int it = wave; it < (kU / 16) * RG; it += NW) {
     const int ct = it / RG, r0 = 16 * RTI * (it % RG);
@@ -343,13 +378,16 @@ inline size_t fwd_lds_floats(const MlpDims& d) {
   return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + d.ld3 + 4);
 }
 
+// act (optional, no camera embedding): each row's relu(z1) and layer-3 input,
+// 256 floats, kept for the backward (load_act).
 template <int TM, int NW, bool BF, bool STD>
 __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpPack pk,
                                                      const float* __restrict__ feats,
                                                      const float* __restrict__ dirs,
                                                      const uint32_t* __restrict__ cams,
                                                      int64_t M, int64_t rpr,
-                                                     float* __restrict__ rgb) {
+                                                     float* __restrict__ rgb,
+                                                     float* __restrict__ act) {
   const MlpDims d = STD ? std_dims<BF>(d_arg.ncam) : d_arg;
   extern __shared__ float smem[];
   // regions: A = Fs, later H1; B = X, later H2; small: Ds, R3, Cs
@@ -365,6 +403,16 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   // layer 1 reads X (regB) and writes H1 over Fs (regA, dead after layer 0);
   // layer 2 reads H1 and writes H2 over X (dead after layer 1)
   forward_hidden<TM, NW, BF, TM / 16>(d, w, op, regA, regB, nullptr, regA, regB, regB, Ds, Cs);
+  if (act && d.ncam == 0) {
+    for (int e = threadIdx.x; e < TM * 64; e += blockDim.x) {
+      const int i = e >> 6, c4 = e & 63;
+      if (row0 + i < M) {
+        const float* src = c4 < 32 ? regA + i * d.ldU + 4 * c4 : regB + i * d.ldU + 4 * (c4 - 32);
+        *reinterpret_cast<float4*>(act + (row0 + i) * 256 + 4 * c4) =
+            *reinterpret_cast<const float4*>(src);
+      }
+    }
+  }
   forward_out<TM, NW, BF>(d, w, op, regB, R3);
   for (int e = threadIdx.x; e < TM * 3; e += blockDim.x) {
     const int i = e / 3, c = e % 3;
@@ -555,7 +603,9 @@ constexpr int kCnt3 = (8 + kBwdNW - 1) / kBwdNW, kLSlot3 = 0;
 constexpr int kCnt0 = (20 + kBwdNW - 1) / kBwdNW, kLSlot0 = kLSlot3 + kCnt3;
 constexpr int kLdsSlots = kLSlot0 + kCnt0;
 
-template <int TM, int NW, bool BF, int NT, bool STD, bool FILM>
+// ACT: H1, H2 and the output come from the forward kernel's act / rgb (no
+// camera embedding): only layer 0 and the encoding are recomputed.
+template <int TM, int NW, bool BF, int NT, bool STD, bool FILM, bool ACT = false>
 __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpPack pk, GradPack gp_arg,
                                                      const float* __restrict__ feats,
                                                      const float* __restrict__ dirs,
@@ -563,7 +613,9 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
                                                      const float* __restrict__ grgb, int64_t M,
                                                      int64_t rpr, float* __restrict__ gfeat,
                                                      float* __restrict__ partials,
-                                                     float* __restrict__ gembed) {
+                                                     float* __restrict__ gembed,
+                                                     const float* __restrict__ act,
+                                                     const float* __restrict__ rgb_in) {
   constexpr int RT = TM / 16;
   static_assert(TM <= 64, "the FiLM camera check reads one tile row per lane");
   // row tiles per work item of the strip phases: all of them (the B strip is
@@ -616,6 +668,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     const int64_t row0 = tile * TM;
     __syncthreads();
     load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
+    if constexpr (ACT) load_act<TM, NW * 64>(d, act, rgb_in, row0, M, H1, H2, Z3);
     __syncthreads();
     if (FILM && d.ncam > 0 && wave == 0) {
       // read before the next use: forward_hidden synchronises the workgroup
@@ -634,8 +687,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
         if (tc >= 0) Run[1] = tc;
       }
     }
-    forward_hidden<TM, NW, BF, BRTI>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
-    forward_out<TM, NW, BF>(d, w, op, H2, Z3);
+    forward_hidden<TM, NW, BF, BRTI, !ACT>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
+    if constexpr (!ACT) forward_out<TM, NW, BF>(d, w, op, H2, Z3);
     // dZ3 = grgb * s * (1 - s), zero outside the 3 colours and the valid rows
     for (int e = threadIdx.x; e < TM * d.Z3W; e += blockDim.x) {
       const int i = e / d.Z3W, c = e % d.Z3W;
@@ -867,6 +920,11 @@ void launch_mlp_bwd_std(bool bf, unsigned nblk, size_t lds, hipStream_t st, cons
                         const float* feats, const float* dirs, const uint32_t* cams,
                         const float* grgb, int64_t M, int64_t rpr, float* gfeat,
                         float* partials, float* gembed);
+void launch_mlp_bwd_std_act(bool bf, unsigned nblk, size_t lds, hipStream_t st,
+                            const MlpDims& d, const MlpW& w, const MlpPack& pk,
+                            const GradPack& gp, const float* feats, const float* dirs,
+                            const float* grgb, int64_t M, int64_t rpr, float* gfeat,
+                            float* partials, const float* act, const float* rgb);
 void launch_mlp_bwd_std_film(bool bf, unsigned nblk, size_t lds, hipStream_t st,
                              const MlpDims& d, const MlpW& w, const MlpPack& pk,
                              const GradPack& gp, const float* feats, const float* dirs,

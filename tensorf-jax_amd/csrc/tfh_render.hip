@@ -21,14 +21,17 @@
 namespace tfh {
 
 int32_t check_mlp(const tfh_mlp* p, const char* who);
+// act (optional): (M, 256) activations the forward keeps for the backward,
+// which then also reads the forward's rgb; only for mlp_keeps_activations().
 int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
                      int64_t M, int64_t rows_per_ray, float* rgb, int32_t precision, void* ws,
-                     size_t ws_bytes, hipStream_t st);
+                     size_t ws_bytes, hipStream_t st, float* act = nullptr);
 size_t mlp_bwd_workspace(const tfh_mlp* p, int64_t M, int32_t precision);
 int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
                      int64_t M, int64_t rows_per_ray, const float* grgb, float* gfeat,
                      const tfh_mlp_grad* g, int32_t precision, void* ws, size_t ws_bytes,
-                     hipStream_t st);
+                     hipStream_t st, const float* act = nullptr, const float* rgb = nullptr);
+bool mlp_keeps_activations(const tfh_mlp* p, int32_t precision);
 int32_t adam_launch(float* p, float* g, float* m, float* v, int64_t n, int64_t split, float lr_lo,
                     float lr_hi, float decay, float b1, float b2, float eps, int64_t count,
                     int zero_grad, hipStream_t st);
@@ -476,6 +479,7 @@ struct RenderWs {
   float* dfeat;  // (M, F)
   void* mlp_ws;
   size_t mlp_ws_bytes;
+  float* act = nullptr;  // (M, 256) MLP activations kept for the backward, or null
 };
 
 static int32_t check_cfg(const tfh_render_config* cfg, const char* who) {
@@ -523,6 +527,23 @@ static size_t carve_render(Carve& cv, const tfh_render_config* cfg, const tfh_ml
   return cv.off;
 }
 
+// The fused training steps keep the MLP's hidden activations between its
+// forward and backward (the backward then recomputes only layer 0 and the
+// encoding) when the shape allows it and the workspace has the room: 1 KB per
+// shaded sample past the render workspace.
+static size_t act_bytes(const tfh_render_config* cfg, const tfh_mlp* mlp, int64_t R) {
+  if (!mlp_keeps_activations(mlp, cfg->mlp_precision)) return 0;
+  return align_up((size_t)R * cfg->appearance_samples_per_ray * 256 * sizeof(float), 256);
+}
+
+static void take_act(Carve& cv, const tfh_render_config* cfg, const tfh_mlp* mlp, int64_t R,
+                     RenderWs& ws) {
+  ws.act = nullptr;
+  const size_t bytes = act_bytes(cfg, mlp, R);
+  if (bytes == 0 || !cv.ok || cv.off + bytes > cv.cap) return;
+  ws.act = cv.take<float>((int64_t)(bytes / sizeof(float)));
+}
+
 static RayIn make_rayin(const tfh_render_config* cfg, const float* aabb, const float* o,
                         const float* d, int64_t R, const float* jitter, const float* base_ts,
                         const float* base_steps, const float* gumbel) {
@@ -568,7 +589,7 @@ static int32_t render_forward(const tfh_vm* density, const tfh_vm* appearance, c
   {
     ProfScope ps("mlp_fwd", st);
     TFH_TRY(mlp_fwd_rows(mlp, ws.feats, in.d, cams, M, k, ws.rgb, cfg->mlp_precision, ws.mlp_ws,
-                         ws.mlp_ws_bytes, st));
+                         ws.mlp_ws_bytes, st, ws.act));
   }
   ProfScope ps("composite_rgb", st);
   hipLaunchKernelGGL(k_composite_rgb, dim3((unsigned)R), dim3(64), 0, st, ws.d.w, ws.d.idx, ws.rgb,
@@ -591,7 +612,7 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
   }
   TFH_TRY(check_launch("render bwd: compositing"));
   TFH_TRY(mlp_bwd_rows(mlp, ws.feats, dirs, cams, M, k, ws.dc, ws.dfeat, mgrad, cfg->mlp_precision,
-                       ws.mlp_ws, ws.mlp_ws_bytes, st));
+                       ws.mlp_ws, ws.mlp_ws_bytes, st, ws.act, ws.act ? ws.rgb : nullptr));
   RaySamples src{ws.d.pts, ws.d.idx, k, N};
   ProfScope ps("appearance_scatter", st);
   return launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, *agrad, st);
@@ -710,6 +731,17 @@ size_t tfh_render_workspace_size(const tfh_render_config* cfg, const tfh_mlp* ml
   return carve_render(cv, cfg, mlp, 3 * appearance_channels, num_rays, ws);
 }
 
+size_t tfh_train_step_workspace_size(const tfh_render_config* cfg, const tfh_mlp* mlp,
+                                     int64_t appearance_channels, int64_t num_rays) {
+  if (check_cfg(cfg, "tfh_train_step_workspace_size") != TFH_OK || num_rays < 0 || !mlp ||
+      check_mlp(mlp, "tfh_train_step_workspace_size") != TFH_OK)
+    return 0;
+  Carve cv(nullptr, 0);
+  RenderWs ws;
+  return carve_render(cv, cfg, mlp, 3 * appearance_channels, num_rays, ws) +
+         act_bytes(cfg, mlp, num_rays);
+}
+
 int32_t tfh_render_fwd(const tfh_vm* density, const tfh_vm* appearance, const tfh_mlp* mlp,
                        const tfh_render_config* cfg, const float* aabb, const float* origins,
                        const float* directions, const uint32_t* camera_indices, int64_t num_rays,
@@ -790,6 +822,7 @@ int32_t tfh_train_step(const tfh_vm* density, const tfh_vm* appearance, const tf
   if (!cv.ok)
     return fail(TFH_ERR_WORKSPACE, "tfh_train_step: workspace of %zu bytes, need %zu",
                 workspace_bytes, cv.off);
+  take_act(cv, cfg, mlp, num_rays, ws);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mse_out, (int64_t)1, 0.0f);
   TFH_TRY(check_launch("train step: clear mse"));
   RayIn in = make_rayin(cfg, aabb, origins, directions, num_rays, jitter, base_ts, base_steps,
@@ -817,7 +850,7 @@ size_t tfh_train_step_proposal_workspace_size(const tfh_render_config* cfg, cons
   RenderWs ws;
   carve_render(cv, cfg, mlp, 3 * appearance_channels, num_rays, ws);
   cv.take<float>(num_rays * (cfg->density_samples_per_ray + 1));
-  return cv.off + prop_workspace_bytes(pc, field, num_rays);
+  return cv.off + prop_workspace_bytes(pc, field, num_rays) + act_bytes(cfg, mlp, num_rays);
 }
 
 int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
@@ -873,6 +906,7 @@ int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
   if (!cv.ok)
     return fail(TFH_ERR_WORKSPACE, "%s: workspace of %zu bytes, need %zu", who, workspace_bytes,
                 cv.off);
+  take_act(cv, cfg, mlp, R, ws);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mse_out, (int64_t)1, 0.0f);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, prop_loss_out, (int64_t)1, 0.0f);
   TFH_TRY(check_launch("train step: clear losses"));

@@ -111,6 +111,7 @@ _SIGS = {
     "tfh_feature_mlp_bwd": (c_i32, [P(Mlp), c_p, c_p, c_p, c_i64, c_p, c_p, P(MlpGrad), c_i32,
                                     c_p, c_sz, c_p]),
     "tfh_render_workspace_size": (c_sz, [P(RenderConfig), P(Mlp), c_i64, c_i64]),
+    "tfh_train_step_workspace_size": (c_sz, [P(RenderConfig), P(Mlp), c_i64, c_i64]),
     "tfh_render_fwd": (c_i32, [P(Vm), P(Vm), P(Mlp), P(RenderConfig), c_p, c_p, c_p, c_p, c_i64,
                                c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
     "tfh_render_bwd": (c_i32, [P(Vm), P(Vm), P(Mlp), P(RenderConfig), c_p, c_p, c_p, c_p, c_i64,

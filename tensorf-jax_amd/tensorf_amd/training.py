@@ -236,8 +236,16 @@ class TrainState:
         lp = self.learnable_params
         cfg = self.render_config()
         env = render._RenderEnv(self.appearance_mlp, lp, self.aabb, rays,
-                                render._seed(self.prng_key, self.step), cfg, self._ws)
-        self._ws = env.ws  # reused by later steps of the same shape
+                                render._seed(self.prng_key, self.step), cfg, self._ws,
+                                alloc_workspace=False)
+        # the training step's workspace also keeps the MLP activations between
+        # its forward and backward when the shape allows it; reused by later
+        # steps of the same shape
+        ca = lp.appearance_tensor.stacked_single_vm.channel_dim()
+        need = int(_lib.lib.tfh_train_step_workspace_size(
+            ctypes.byref(env.cfg), ctypes.byref(env.cmlp), ca, env.r))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(max(need, 1), dtype=torch.uint8, device=env.device)
         grads_lp = (self._vm(self.flat.grads, 0), self._vm(self.flat.grads, 2),
                     self._mlp_params(self.flat.grads))
         dg = _lib.VmGrad(_lib.ptr(grads_lp[0].stacked_single_vm.plane),
@@ -256,7 +264,7 @@ class TrainState:
                   _lib.ptr(env.gumbel), ctypes.byref(dg), ctypes.byref(ag), ctypes.byref(mg),
                   ctypes.byref(bufs), self.config.optimizer.lr_init_tensor,
                   self.config.optimizer.lr_init_mlp, decay, self.adam_count, _lib.ptr(self._mse),
-                  _lib.ptr(env.ws), env.ws.numel(), _lib.stream())
+                  _lib.ptr(self._ws), self._ws.numel(), _lib.stream())
         self.step += 1
         logs = {"mse": self._mse, "lr_tensor": decay * self.config.optimizer.lr_init_tensor,
                 "lr_mlp": decay * self.config.optimizer.lr_init_mlp}

@@ -163,3 +163,40 @@ def test_proposal64_matches_uniform512_at_a_fraction_of_the_samples():
     assert res["proposal-64"] >= res["uniform-512"] - 1.0
     assert held_psnr["proposal-64"] >= held_psnr["uniform-512"] - 1.0
     assert sec["proposal-64"] * 3.0 <= sec["uniform-512"]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_train_step_kept_activations_match_recompute(monkeypatch, precision):
+    """The fused training step keeps the MLP's hidden activations between its
+    forward and backward when the workspace has the room
+    (tfh_train_step_workspace_size) and recomputes them with a render-sized
+    workspace; both give the same step: the same MSE bit for bit, and the same
+    Adam update except where the order of the scatter atomics tips a
+    near-zero gradient (at most 1e-3 of the parameters off by > 1e-3 lr)."""
+    log(f"case: train step, kept activations vs recompute, {precision}")
+    from tensorf_amd import _lib, training
+    _, c2ws, data = _small_dataset(views=4, res=32)
+    out = []
+    for keep in (True, False):
+        cfg = dataclasses.replace(_config(g=24, n=64), mlp_precision=precision)
+        st = training.TrainState.initialize(cfg, 24, prng_key=5, num_cameras=4)
+        p0 = st.flat.params.clone()
+        if not keep:
+            monkeypatch.setattr(_lib.lib, "tfh_train_step_workspace_size",
+                                _lib.lib.tfh_render_workspace_size)
+        st, logs = st.training_step(next(data.minibatches(1024, seed=2)))
+        torch.cuda.synchronize()
+        out.append((float(logs["mse"].item()), (st.flat.params - p0).cpu().numpy(),
+                    st._ws.numel(), st.flat.c_struct().tensor_count))
+        monkeypatch.undo()
+    (mse_a, upd_a, ws_a, split), (mse_b, upd_b, ws_b, _) = out
+    log(f"  workspace {ws_a} vs {ws_b} bytes, mse {mse_a:.9e} vs {mse_b:.9e}")
+    assert ws_a > ws_b
+    assert abs(mse_a - mse_b) <= 1e-5 * mse_b
+    opt = training.OptimizerConfig()
+    for name, sl, lr in (("factors", slice(0, split), opt.lr_init_tensor),
+                         ("mlp", slice(split, None), opt.lr_init_mlp)):
+        e = report(f"update {name}", upd_a[sl], upd_b[sl])
+        frac_bad = float(np.mean(e > 1e-3 * lr))
+        log(f"  fraction of {name} updates off by >1e-3 lr: {frac_bad:.2e}")
+        assert frac_bad <= 1e-3, name
