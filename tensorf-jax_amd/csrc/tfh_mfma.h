@@ -99,12 +99,21 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
   const int r = lane & 15, kq = lane >> 4;
   constexpr int NS = KMAX / 16;
   for (int kc = 0; kc < K; kc += KMAX) {  // one pass for K <= KMAX
+    // B chunks stream through a ring PD deep: chunk i + PD is requested
+    // before chunk i is multiplied, so PD L2 round trips overlap the products.
+    // The scheduling barriers pin that order; left to itself the scheduler
+    // sinks each load next to its use under register pressure (one round trip
+    // per chunk), and hoisting all NS loads costs NS * 4 registers.
+    constexpr int PD = NS < 4 ? NS : 4;
     float4 bb[NS];
 #pragma unroll
-    for (int i = 0; i < NS; ++i)
+    for (int i = 0; i < PD; ++i)
       if (kc + 16 * i < K) bb[i] = b.load4(kc + 16 * i + 4 * kq, col0 + r);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
+      if (i + PD < NS && kc + 16 * (i + PD) < K)
+        bb[i + PD] = b.load4(kc + 16 * (i + PD) + 4 * kq, col0 + r);
       if (kc + 16 * i < K) {
 #pragma unroll
         for (int rt = 0; rt < NR; ++rt) {
@@ -116,6 +125,7 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
           acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, bb[i].w, acc[rt], 0, 0, 0);
         }
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }

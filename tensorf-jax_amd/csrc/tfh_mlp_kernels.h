@@ -202,36 +202,84 @@ __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
   }
 }
 
-// Activations the forward kernel kept for the backward: per row 256 floats,
-// relu(z1) then the layer-3 input (no camera embedding); plus the row's
-// sigmoid output from rgb. Loaded into H1, H2 (stride ldU) and Z3 (stride
-// ld3), zero past M; the float4 loads are all issued before the stores.
+// The whole tile of the activation-reuse backward in one pass: feature rows
+// (Fs, zero padded to FP), view directions (Ds), the activations the forward
+// kernel kept (per row 256 floats: relu(z1), then the layer-3 input; into H1,
+// H2) and dZ3 = grgb * s * (1 - s) from the kept output s (Z3, zero past the
+// 3 colours). Each source is a buffer resource over the tile's valid rows, so
+// an offset is 32 bits and rows past M read 0 with no branch; every load is
+// issued before the first LDS store and the tile pays one memory latency.
+// Requires F % 4 == 0 and 0 < rpr < 2^31.
 template <int TM, int NT>
-__device__ void load_act(const MlpDims& d, const float* __restrict__ act,
-                         const float* __restrict__ rgb, int64_t row0, int64_t M, float* H1,
-                         float* H2, float* Z3) {
-  constexpr int kPer = (TM * 64 + NT - 1) / NT;  // float4 per thread, NT threads
-  float4 v[kPer];
+__device__ void load_tile_act(const MlpDims& d, const float* __restrict__ feats,
+                              const float* __restrict__ dirs, const float* __restrict__ act,
+                              const float* __restrict__ rgb, const float* __restrict__ grgb,
+                              int64_t row0, int64_t M, int64_t rpr, float* Fs, float* Ds,
+                              float* H1, float* H2, float* Z3) {
+  typedef int i32x4 __attribute__((ext_vector_type(4)));
+  constexpr int kA4 = TM * 64 / NT;  // float4 of activations per thread
+  static_assert(TM * 64 % NT == 0 && TM * 3 <= NT, "tile shape");
+  const int nv = (int)min((int64_t)TM, M - row0);
+  const int64_t ray0 = row0 / rpr;
+  const unsigned rem0 = (unsigned)(row0 - ray0 * rpr);
+  const int nray = (int)((row0 + nv - 1) / rpr - ray0) + 1;
+  // the feature base goes through an empty asm: a load from a known
+  // read-only argument counts as invariant, and the register allocator then
+  // re-issues it at its use (after the other loads' waits) instead of keeping it
+  const float* fbase = feats + row0 * d.F;
+  asm volatile("" : "+s"(fbase));
+  const auto rf = buffer_rsrc(fbase, nv * d.F * 4);
+  const auto ra = buffer_rsrc(act + row0 * 256, nv * 1024);
+  const auto rs = buffer_rsrc(rgb + row0 * 3, nv * 12);
+  const auto rg = buffer_rsrc(grgb + row0 * 3, nv * 12);
+  const auto rd = buffer_rsrc(dirs + ray0 * 3, nray * 12);
+  const int fp4 = d.FP / 4, f4 = d.F / 4, n4 = TM * fp4;
+  const int kF4 = (n4 + NT - 1) / NT;  // constant for the specialised shape
+  const int t = threadIdx.x;
+  // issue
+  i32x4 vf[(TM * 160 / 4 + NT - 1) / NT];  // room for FP <= 160
+  i32x4 va[kA4];
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int e = threadIdx.x + k * NT;
-    const int i = e >> 6, c4 = e & 63;
-    v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (e < TM * 64 && row0 + i < M)
-      v[k] = *reinterpret_cast<const float4*>(act + (row0 + i) * 256 + 4 * c4);
-  }
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int e = threadIdx.x + k * NT;
-    const int i = e >> 6, c4 = e & 63;
-    if (e < TM * 64) {
-      float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
-      *reinterpret_cast<float4*>(dst) = v[k];
+  for (int k = 0; k < (TM * 160 / 4 + NT - 1) / NT; ++k) {
+    if (k < kF4) {
+      const int e = t + k * NT, i = e / fp4, c4 = e - i * fp4;
+      const int off = (e < n4 && c4 < f4) ? (i * d.F + 4 * c4) * 4 : kOOB;
+      vf[k] = __builtin_amdgcn_raw_buffer_load_b128(rf, off, 0, 0);
     }
   }
-  for (int e = threadIdx.x; e < TM * 3; e += NT) {
-    const int i = e / 3, c = e % 3;
-    Z3[i * d.ld3 + c] = row0 + i < M ? rgb[(row0 + i) * 3 + c] : 0.0f;
+#pragma unroll
+  for (int k = 0; k < kA4; ++k)
+    va[k] = __builtin_amdgcn_raw_buffer_load_b128(ra, (t + k * NT) * 16, 0, 0);
+  const int i3 = t / 3, c3 = t - 3 * i3;
+  const bool own3 = t < TM * 3;
+  const float s = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, own3 ? t * 4 : kOOB, 0, 0));
+  const float g = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rg, own3 ? t * 4 : kOOB, 0, 0));
+  const unsigned rel = (rem0 + (unsigned)i3) / (unsigned)rpr;  // ray of row i3 after ray0
+  const float dv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+      rd, (own3 && i3 < nv) ? (int)(rel * 3 + c3) * 4 : kOOB, 0, 0));
+  // keeps the loads here: a load whose only use is a predicated store would
+  // otherwise be sunk into that branch, after the waits of the others
+  asm volatile("" ::: "memory");
+  // store
+#pragma unroll
+  for (int k = 0; k < (TM * 160 / 4 + NT - 1) / NT; ++k) {
+    const int e = t + k * NT, i = e / fp4, c4 = e - i * fp4;
+    if (k < kF4 && e < n4)
+      *reinterpret_cast<float4*>(Fs + i * d.ldF + 4 * c4) = __builtin_bit_cast(float4, vf[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kA4; ++k) {
+    const int e = t + k * NT, i = e >> 6, c4 = e & 63;
+    float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
+    *reinterpret_cast<float4*>(dst) = __builtin_bit_cast(float4, va[k]);
+  }
+  if (own3) {
+    Z3[i3 * d.ld3 + c3] = g * s * (1.0f - s);
+    Ds[i3 * 3 + c3] = dv;
+  }
+  for (int e = t; e < TM * (d.Z3W - 3); e += NT) {
+    const int i = e / (d.Z3W - 3), c = 3 + e % (d.Z3W - 3);
+    Z3[i * d.ld3 + c] = 0.0f;
   }
 }
 
@@ -379,7 +427,7 @@ inline size_t fwd_lds_floats(const MlpDims& d) {
 }
 
 // act (optional, no camera embedding): each row's relu(z1) and layer-3 input,
-// 256 floats, kept for the backward (load_act).
+// 256 floats, kept for the backward (load_tile_act).
 template <int TM, int NW, bool BF, bool STD>
 __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpPack pk,
                                                      const float* __restrict__ feats,
@@ -667,8 +715,12 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t row0 = tile * TM;
     __syncthreads();
-    load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
-    if constexpr (ACT) load_act<TM, NW * 64>(d, act, rgb_in, row0, M, H1, H2, Z3);
+    if constexpr (ACT) {
+      load_tile_act<TM, NW * 64>(d, feats, dirs, act, rgb_in, grgb, row0, M, rpr, Fs, Ds, H1, H2,
+                                 Z3);
+    } else {
+      load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
+    }
     __syncthreads();
     if (FILM && d.ncam > 0 && wave == 0) {
       // read before the next use: forward_hidden synchronises the workgroup
@@ -690,17 +742,20 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     forward_hidden<TM, NW, BF, BRTI, !ACT>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
     if constexpr (!ACT) forward_out<TM, NW, BF>(d, w, op, H2, Z3);
     // dZ3 = grgb * s * (1 - s), zero outside the 3 colours and the valid rows
-    for (int e = threadIdx.x; e < TM * d.Z3W; e += blockDim.x) {
-      const int i = e / d.Z3W, c = e % d.Z3W;
-      const int64_t row = row0 + i;
-      float v = 0.0f;
-      if (c < 3 && row < M) {
-        const float s = Z3[i * d.ld3 + c];
-        v = grgb[row * 3 + c] * s * (1.0f - s);
+    // (loaded with the tile when the activations are kept)
+    if constexpr (!ACT) {
+      for (int e = threadIdx.x; e < TM * d.Z3W; e += blockDim.x) {
+        const int i = e / d.Z3W, c = e % d.Z3W;
+        const int64_t row = row0 + i;
+        float v = 0.0f;
+        if (c < 3 && row < M) {
+          const float s = Z3[i * d.ld3 + c];
+          v = grgb[row * 3 + c] * s * (1.0f - s);
+        }
+        Z3[i * d.ld3 + c] = v;
       }
-      Z3[i * d.ld3 + c] = v;
+      __syncthreads();
     }
-    __syncthreads();
     // dW3 (U x 3) += H2^T dZ3 ; db3
     wgrad_phase<TM, NW, BF, NT, kLSlot3, kCnt3, true>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3, d.ld3,
                                                       part, gp.w3, kU, 3, Wl);
