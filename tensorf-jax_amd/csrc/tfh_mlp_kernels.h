@@ -202,22 +202,22 @@ __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
   }
 }
 
-// The whole tile of the activation-reuse backward in one pass: feature rows
-// (Fs, zero padded to FP), view directions (Ds), the activations the forward
+// A tile in one pass, without camera indices: feature rows (Fs, zero padded
+// to FP) and view directions (Ds); with ACT also the activations the forward
 // kernel kept (per row 256 floats: relu(z1), then the layer-3 input; into H1,
 // H2) and dZ3 = grgb * s * (1 - s) from the kept output s (Z3, zero past the
 // 3 colours). Each source is a buffer resource over the tile's valid rows, so
 // an offset is 32 bits and rows past M read 0 with no branch; every load is
 // issued before the first LDS store and the tile pays one memory latency.
 // Requires F % 4 == 0 and 0 < rpr < 2^31.
-template <int TM, int NT>
-__device__ void load_tile_act(const MlpDims& d, const float* __restrict__ feats,
-                              const float* __restrict__ dirs, const float* __restrict__ act,
-                              const float* __restrict__ rgb, const float* __restrict__ grgb,
-                              int64_t row0, int64_t M, int64_t rpr, float* Fs, float* Ds,
-                              float* H1, float* H2, float* Z3) {
+template <int TM, int NT, bool ACT>
+__device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats,
+                               const float* __restrict__ dirs, const float* __restrict__ act,
+                               const float* __restrict__ rgb, const float* __restrict__ grgb,
+                               int64_t row0, int64_t M, int64_t rpr, float* Fs, float* Ds,
+                               float* H1, float* H2, float* Z3) {
   typedef int i32x4 __attribute__((ext_vector_type(4)));
-  constexpr int kA4 = TM * 64 / NT;  // float4 of activations per thread
+  constexpr int kA4 = ACT ? TM * 64 / NT : 1;  // float4 of activations per thread
   static_assert(TM * 64 % NT == 0 && TM * 3 <= NT, "tile shape");
   const int nv = (int)min((int64_t)TM, M - row0);
   const int64_t ray0 = row0 / rpr;
@@ -229,9 +229,9 @@ __device__ void load_tile_act(const MlpDims& d, const float* __restrict__ feats,
   const float* fbase = feats + row0 * d.F;
   asm volatile("" : "+s"(fbase));
   const auto rf = buffer_rsrc(fbase, nv * d.F * 4);
-  const auto ra = buffer_rsrc(act + row0 * 256, nv * 1024);
-  const auto rs = buffer_rsrc(rgb + row0 * 3, nv * 12);
-  const auto rg = buffer_rsrc(grgb + row0 * 3, nv * 12);
+  const auto ra = buffer_rsrc(ACT ? act + row0 * 256 : feats, ACT ? nv * 1024 : 0);
+  const auto rs = buffer_rsrc(ACT ? rgb + row0 * 3 : feats, ACT ? nv * 12 : 0);
+  const auto rg = buffer_rsrc(ACT ? grgb + row0 * 3 : feats, ACT ? nv * 12 : 0);
   const auto rd = buffer_rsrc(dirs + ray0 * 3, nray * 12);
   const int fp4 = d.FP / 4, f4 = d.F / 4, n4 = TM * fp4;
   const int kF4 = (n4 + NT - 1) / NT;  // constant for the specialised shape
@@ -247,13 +247,16 @@ __device__ void load_tile_act(const MlpDims& d, const float* __restrict__ feats,
       vf[k] = __builtin_amdgcn_raw_buffer_load_b128(rf, off, 0, 0);
     }
   }
-#pragma unroll
-  for (int k = 0; k < kA4; ++k)
-    va[k] = __builtin_amdgcn_raw_buffer_load_b128(ra, (t + k * NT) * 16, 0, 0);
+  float s = 0.0f, g = 0.0f;
   const int i3 = t / 3, c3 = t - 3 * i3;
   const bool own3 = t < TM * 3;
-  const float s = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, own3 ? t * 4 : kOOB, 0, 0));
-  const float g = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rg, own3 ? t * 4 : kOOB, 0, 0));
+  if constexpr (ACT) {
+#pragma unroll
+    for (int k = 0; k < kA4; ++k)
+      va[k] = __builtin_amdgcn_raw_buffer_load_b128(ra, (t + k * NT) * 16, 0, 0);
+    s = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, own3 ? t * 4 : kOOB, 0, 0));
+    g = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rg, own3 ? t * 4 : kOOB, 0, 0));
+  }
   const unsigned rel = (rem0 + (unsigned)i3) / (unsigned)rpr;  // ray of row i3 after ray0
   const float dv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
       rd, (own3 && i3 < nv) ? (int)(rel * 3 + c3) * 4 : kOOB, 0, 0));
@@ -267,19 +270,19 @@ __device__ void load_tile_act(const MlpDims& d, const float* __restrict__ feats,
     if (k < kF4 && e < n4)
       *reinterpret_cast<float4*>(Fs + i * d.ldF + 4 * c4) = __builtin_bit_cast(float4, vf[k]);
   }
+  if (own3) Ds[i3 * 3 + c3] = dv;
+  if constexpr (ACT) {
 #pragma unroll
-  for (int k = 0; k < kA4; ++k) {
-    const int e = t + k * NT, i = e >> 6, c4 = e & 63;
-    float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
-    *reinterpret_cast<float4*>(dst) = __builtin_bit_cast(float4, va[k]);
-  }
-  if (own3) {
-    Z3[i3 * d.ld3 + c3] = g * s * (1.0f - s);
-    Ds[i3 * 3 + c3] = dv;
-  }
-  for (int e = t; e < TM * (d.Z3W - 3); e += NT) {
-    const int i = e / (d.Z3W - 3), c = 3 + e % (d.Z3W - 3);
-    Z3[i * d.ld3 + c] = 0.0f;
+    for (int k = 0; k < kA4; ++k) {
+      const int e = t + k * NT, i = e >> 6, c4 = e & 63;
+      float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
+      *reinterpret_cast<float4*>(dst) = __builtin_bit_cast(float4, va[k]);
+    }
+    if (own3) Z3[i3 * d.ld3 + c3] = g * s * (1.0f - s);
+    for (int e = t; e < TM * (d.Z3W - 3); e += NT) {
+      const int i = e / (d.Z3W - 3), c = 3 + e % (d.Z3W - 3);
+      Z3[i * d.ld3 + c] = 0.0f;
+    }
   }
 }
 
@@ -427,7 +430,7 @@ inline size_t fwd_lds_floats(const MlpDims& d) {
 }
 
 // act (optional, no camera embedding): each row's relu(z1) and layer-3 input,
-// 256 floats, kept for the backward (load_tile_act).
+// 256 floats, kept for the backward (load_tile_rows).
 template <int TM, int NW, bool BF, bool STD>
 __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpPack pk,
                                                      const float* __restrict__ feats,
@@ -446,7 +449,11 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   int* Cs = reinterpret_cast<int*>(R3 + TM * d.ld3);
   const Operands<BF> op{d, w, pk};
   const int64_t row0 = (int64_t)blockIdx.x * TM;
-  load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, regA, Ds, Cs);
+  if (STD && d.ncam == 0)
+    load_tile_rows<TM, NW * 64, false>(d, feats, dirs, nullptr, nullptr, nullptr, row0, M, rpr,
+                                       regA, Ds, nullptr, nullptr, nullptr);
+  else
+    load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, regA, Ds, Cs);
   __syncthreads();
   // layer 1 reads X (regB) and writes H1 over Fs (regA, dead after layer 0);
   // layer 2 reads H1 and writes H2 over X (dead after layer 1)
@@ -716,8 +723,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     const int64_t row0 = tile * TM;
     __syncthreads();
     if constexpr (ACT) {
-      load_tile_act<TM, NW * 64>(d, feats, dirs, act, rgb_in, grgb, row0, M, rpr, Fs, Ds, H1, H2,
-                                 Z3);
+      load_tile_rows<TM, NW * 64, true>(d, feats, dirs, act, rgb_in, grgb, row0, M, rpr, Fs, Ds,
+                                        H1, H2, Z3);
     } else {
       load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
     }
