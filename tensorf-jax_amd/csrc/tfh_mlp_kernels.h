@@ -426,7 +426,8 @@ __device__ void forward_out(const MlpDims& d, const MlpW& w, const Operands<BF>&
 // ---- forward kernel -----------------------------------------------------------
 template <int TM>
 inline size_t fwd_lds_floats(const MlpDims& d) {
-  return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + d.ld3 + 4);
+  // the layer-3 output reuses region A (H1 is dead by then)
+  return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + 4);
 }
 
 // act (optional, no camera embedding): each row's relu(z1) and layer-3 input,
@@ -441,12 +442,14 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
                                                      float* __restrict__ act) {
   const MlpDims d = STD ? std_dims<BF>(d_arg.ncam) : d_arg;
   extern __shared__ float smem[];
-  // regions: A = Fs, later H1; B = X, later H2; small: Ds, R3, Cs
+  // regions: A = Fs, later H1, last R3 (the layer-3 output); B = X, later
+  // H2; small: Ds, Cs. 40 KB per workgroup for the FP32 standard shape: four
+  // workgroups share a CU.
   float* regA = smem;
   float* regB = regA + TM * std::max(d.ldF, d.ldU);
   float* Ds = regB + TM * std::max(d.ldE, d.ldU);
-  float* R3 = Ds + TM * 4;
-  int* Cs = reinterpret_cast<int*>(R3 + TM * d.ld3);
+  int* Cs = reinterpret_cast<int*>(Ds + TM * 4);
+  float* R3 = regA;
   const Operands<BF> op{d, w, pk};
   const int64_t row0 = (int64_t)blockIdx.x * TM;
   if (STD && d.ncam == 0)
@@ -468,6 +471,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
       }
     }
   }
+  __syncthreads();  // H1 (region A) is read above before R3 overwrites it
   forward_out<TM, NW, BF>(d, w, op, regB, R3);
   for (int e = threadIdx.x; e < TM * 3; e += blockDim.x) {
     const int i = e / 3, c = e % 3;
