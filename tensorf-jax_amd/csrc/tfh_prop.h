@@ -32,6 +32,10 @@ struct PropWs {
   int32_t* nz_idx = nullptr;
   int32_t* nz_blk = nullptr;
   int64_t nz_nblk = 0;
+  // npc copies of the field's plane gradient, then nlc of its line gradient:
+  // the scatter's blocks add into them round-robin, one kernel sums them
+  float* gcopy = nullptr;
+  int npc = 1, nlc = 1;
 };
 
 constexpr int kNzBlock = 1024;

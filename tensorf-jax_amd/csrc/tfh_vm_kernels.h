@@ -121,11 +121,16 @@ inline int32_t launch_vm_gather(const tfh_vm& vm, const Src& src, int64_t M, flo
 
 // VJP: grad (per sample, per channel) -> atomic scatter into the factor
 // gradients. Each wave holds 64/LPS consecutive samples; equal texels among
-// them, in any tap slot, are merged before the atomic (tap_plan).
+// them, in any tap slot, are merged before the atomic (tap_plan). With
+// npc / nlc > 1, grad holds that many consecutive copies of the plane / line
+// gradient and block b adds into copy b % npc / b % nlc: float atomics are
+// performed at the memory side, where adds to one address serialise, and a
+// small field (the proposal's) draws thousands of adds per texel a step.
 template <class Src, bool ROW_MAJOR, int LPS>
 __global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src src, int64_t M,
                                                                const float* __restrict__ gout,
-                                                               tfh_vm_grad grad) {
+                                                               tfh_vm_grad grad, int npc,
+                                                               int nlc) {
   constexpr int SPB = kGatherThreads / LPS;
   constexpr int NQ = 64 / LPS;
   __shared__ float tile[ROW_MAJOR ? 1 : 3 * 64][SPB + 1];
@@ -136,6 +141,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src sr
   const int64_t s = s0 + grp;
   const bool active = s < M;
   const int G = (int)vm.grid_dim, C = (int)vm.channels;
+  grad.plane += (int64_t)(blockIdx.x % npc) * 3 * G * G * C;
+  grad.line += (int64_t)(blockIdx.x % nlc) * 3 * G * C;
   if (!ROW_MAJOR) {
     const int nch = 3 * C;
     for (int e = threadIdx.x; e < nch * SPB; e += kGatherThreads) {
@@ -177,17 +184,18 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src sr
 
 template <class Src, bool ROW_MAJOR>
 inline int32_t launch_vm_scatter(const tfh_vm& vm, const Src& src, int64_t M, const float* gout,
-                                 const tfh_vm_grad& grad, hipStream_t st) {
+                                 const tfh_vm_grad& grad, hipStream_t st, int npc = 1,
+                                 int nlc = 1) {
   const bool narrow = vm.channels <= 8;
   const int spb = kGatherThreads / (narrow ? 8 : 16);
   const int64_t blocks = ceil_div(M, spb);
   TFH_REQUIRE(blocks < (1ll << 31), "vm scatter: too many points (%lld)", (long long)M);
   if (narrow)
     hipLaunchKernelGGL((k_vm_scatter<Src, ROW_MAJOR, 8>), dim3((unsigned)blocks),
-                       dim3(kGatherThreads), 0, st, vm, src, M, gout, grad);
+                       dim3(kGatherThreads), 0, st, vm, src, M, gout, grad, npc, nlc);
   else
     hipLaunchKernelGGL((k_vm_scatter<Src, ROW_MAJOR, 16>), dim3((unsigned)blocks),
-                       dim3(kGatherThreads), 0, st, vm, src, M, gout, grad);
+                       dim3(kGatherThreads), 0, st, vm, src, M, gout, grad, npc, nlc);
   return check_launch("vm scatter");
 }
 
