@@ -415,28 +415,6 @@ int32_t prop_sample(const tfh_vm* field, const tfh_prop_mlp* mlp, const tfh_prop
 // steps on the blob scene), and a zero row contributes nothing to the
 // proposal MLP's gradients or to the factor scatter. The list keeps row order
 // (deterministic): per-block counts, one scan, then ordered writes.
-// The proposal field's gradient copies (carve_prop): cleared before the
-// scatter, then summed into the gradient (+=, each element by one thread).
-__global__ void __launch_bounds__(256) k_zero_copies(float* __restrict__ p, int64_t n) {
-  for (int64_t e = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4; e < n;
-       e += (int64_t)gridDim.x * 256 * 4) {
-    if (e + 4 <= n)
-      *reinterpret_cast<float4*>(p + e) = make_float4(0.f, 0.f, 0.f, 0.f);
-    else
-      for (int64_t j = e; j < n; ++j) p[j] = 0.0f;
-  }
-}
-
-__global__ void __launch_bounds__(256) k_sum_copies(const float* __restrict__ copies, int ncopy,
-                                                    int64_t n, float* __restrict__ dst) {
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * 256) {
-    float sum = 0.0f;
-    for (int k = 0; k < ncopy; ++k) sum += copies[k * n + e];
-    dst[e] += sum;
-  }
-}
-
 __device__ __forceinline__ int block_excl_scan256(int flag, int* s_w, int& total) {
   // 256 threads = 4 waves
   const int lane = lane_id(), wave = threadIdx.x >> 6;
@@ -539,17 +517,12 @@ int32_t prop_loss_bwd(const tfh_vm* field, const tfh_prop_mlp* mlp,
   const int64_t pn = 3 * field->grid_dim * field->grid_dim * field->channels;
   const int64_t ln = 3 * field->grid_dim * field->channels;
   const int64_t ncopy = ws.npc * pn + ws.nlc * ln;
-  hipLaunchKernelGGL(k_zero_copies, dim3((unsigned)std::min<int64_t>(ceil_div(ncopy, 1024), 4096)),
-                     dim3(256), 0, st, ws.gcopy, ncopy);
-  TFH_TRY(check_launch("proposal scatter: clear the copies"));
+  TFH_TRY(zero_copies(ws.gcopy, ncopy, st));
   const tfh_vm_grad priv{ws.gcopy, ws.gcopy + ws.npc * pn};
   TFH_TRY((launch_vm_scatter<CompactedPoints, true>(*field, src, ws.rows, ws.dfeat, priv, st,
                                                     ws.npc, ws.nlc)));
-  hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)std::min<int64_t>(ceil_div(pn, 256), 4096)),
-                     dim3(256), 0, st, ws.gcopy, ws.npc, pn, fgrad->plane);
-  hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)std::min<int64_t>(ceil_div(ln, 256), 4096)),
-                     dim3(256), 0, st, ws.gcopy + ws.npc * pn, ws.nlc, ln, fgrad->line);
-  return check_launch("proposal scatter: sum the copies");
+  TFH_TRY(sum_copies(ws.gcopy, ws.npc, pn, fgrad->plane, st));
+  return sum_copies(ws.gcopy + ws.npc * pn, ws.nlc, ln, fgrad->line, st);
 }
 
 int32_t check_prop_args(const tfh_vm* field, const tfh_prop_mlp* mlp,

@@ -270,7 +270,9 @@ constexpr int kCompBwdWaves = 4;
 
 __global__ void __launch_bounds__(64 * kCompBwdWaves) k_composite_bwd(tfh_vm dvm, BwdIn in,
                                                                      float* __restrict__ dc,
-                                                                     tfh_vm_grad dgrad) {
+                                                                     tfh_vm_grad dgrad, int nlc) {
+  // nlc consecutive copies of the line gradient: ray r adds into copy r % nlc
+  dgrad.line += (int64_t)(blockIdx.x % nlc) * 3 * dvm.grid_dim * dvm.channels;
   extern __shared__ float sm[];
   const int64_t r = blockIdx.x;
   const int lane = lane_id(), wave = threadIdx.x >> 6;
@@ -480,6 +482,10 @@ struct RenderWs {
   void* mlp_ws;
   size_t mlp_ws_bytes;
   float* act = nullptr;  // (M, 256) MLP activations kept for the backward, or null
+  // copies of the density / appearance line gradients for the scatters, or null
+  float* dlcopy = nullptr;
+  float* alcopy = nullptr;
+  int dnlc = 1, anlc = 1;
 };
 
 static int32_t check_cfg(const tfh_render_config* cfg, const char* who) {
@@ -544,6 +550,41 @@ static void take_act(Carve& cv, const tfh_render_config* cfg, const tfh_mlp* mlp
   ws.act = cv.take<float>((int64_t)(bytes / sizeof(float)));
 }
 
+// The training steps' scatters add the factor-line gradients into copies
+// (block b into copy b % n) that one kernel then sums: float atomics are
+// performed at the memory side, where adds to one address serialise, and each
+// line texel draws thousands of adds per step (a plane texel tens). The size
+// queries reserve 8 MB (appearance) + 4 MB (density) for them; up to 256
+// copies of each fit.
+constexpr size_t kALineCopyBytes = 8u << 20, kDLineCopyBytes = 4u << 20;
+
+static size_t line_copy_bytes() {
+  return align_up(kALineCopyBytes, 256) + align_up(kDLineCopyBytes, 256);
+}
+
+static int line_copies(const tfh_vm& vm, size_t budget) {
+  const int64_t bytes = 3 * vm.grid_dim * vm.channels * (int64_t)sizeof(float);
+  return (int)std::min<int64_t>(256, (int64_t)budget / bytes);
+}
+
+static void take_line_copies(Carve& cv, const tfh_vm* density, const tfh_vm* appearance,
+                             RenderWs& ws) {
+  ws.dlcopy = ws.alcopy = nullptr;
+  ws.dnlc = ws.anlc = 1;
+  const int an = line_copies(*appearance, kALineCopyBytes);
+  const int dn = line_copies(*density, kDLineCopyBytes);
+  const int64_t af = an * 3 * appearance->grid_dim * appearance->channels;
+  const int64_t df = dn * 3 * density->grid_dim * density->channels;
+  if (an >= 2 && cv.ok && cv.off + align_up((size_t)af * 4, 256) <= cv.cap) {
+    ws.alcopy = cv.take<float>(af);
+    ws.anlc = an;
+  }
+  if (dn >= 2 && cv.ok && cv.off + align_up((size_t)df * 4, 256) <= cv.cap) {
+    ws.dlcopy = cv.take<float>(df);
+    ws.dnlc = dn;
+  }
+}
+
 static RayIn make_rayin(const tfh_render_config* cfg, const float* aabb, const float* o,
                         const float* d, int64_t R, const float* jitter, const float* base_ts,
                         const float* base_steps, const float* gumbel) {
@@ -605,17 +646,32 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
   const int N = cfg->density_samples_per_ray, k = cfg->appearance_samples_per_ray;
   const int64_t M = R * k;
   BwdIn bi{ws.d.pts, ws.d.sigma, ws.d.delta, ws.d.w, ws.d.e, ws.d.ray, ws.rgb, gout, ws.d.idx, N, k};
+  const int64_t dln = 3 * density->grid_dim * density->channels;
+  const int64_t aln = 3 * appearance->grid_dim * appearance->channels;
   {
     ProfScope ps("composite_bwd_density_scatter", st);
+    tfh_vm_grad dg = *dgrad;
+    if (ws.dlcopy) {
+      TFH_TRY(zero_copies(ws.dlcopy, ws.dnlc * dln, st));
+      dg.line = ws.dlcopy;
+    }
     hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)R), dim3(64 * kCompBwdWaves),
-                       5 * N * sizeof(float), st, *density, bi, ws.dc, *dgrad);
+                       5 * N * sizeof(float), st, *density, bi, ws.dc, dg,
+                       ws.dlcopy ? ws.dnlc : 1);
+    TFH_TRY(check_launch("render bwd: compositing"));
+    if (ws.dlcopy) TFH_TRY(sum_copies(ws.dlcopy, ws.dnlc, dln, dgrad->line, st));
   }
-  TFH_TRY(check_launch("render bwd: compositing"));
   TFH_TRY(mlp_bwd_rows(mlp, ws.feats, dirs, cams, M, k, ws.dc, ws.dfeat, mgrad, cfg->mlp_precision,
                        ws.mlp_ws, ws.mlp_ws_bytes, st, ws.act, ws.act ? ws.rgb : nullptr));
   RaySamples src{ws.d.pts, ws.d.idx, k, N};
   ProfScope ps("appearance_scatter", st);
-  return launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, *agrad, st);
+  if (!ws.alcopy)
+    return launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, *agrad, st);
+  TFH_TRY(zero_copies(ws.alcopy, ws.anlc * aln, st));
+  const tfh_vm_grad ag{agrad->plane, ws.alcopy};
+  TFH_TRY((launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, ag, st, 1,
+                                               ws.anlc)));
+  return sum_copies(ws.alcopy, ws.anlc, aln, agrad->line, st);
 }
 
 static int32_t check_render_args(const tfh_vm* density, const tfh_vm* appearance,
@@ -739,7 +795,7 @@ size_t tfh_train_step_workspace_size(const tfh_render_config* cfg, const tfh_mlp
   Carve cv(nullptr, 0);
   RenderWs ws;
   return carve_render(cv, cfg, mlp, 3 * appearance_channels, num_rays, ws) +
-         act_bytes(cfg, mlp, num_rays);
+         act_bytes(cfg, mlp, num_rays) + line_copy_bytes();
 }
 
 int32_t tfh_render_fwd(const tfh_vm* density, const tfh_vm* appearance, const tfh_mlp* mlp,
@@ -823,6 +879,7 @@ int32_t tfh_train_step(const tfh_vm* density, const tfh_vm* appearance, const tf
     return fail(TFH_ERR_WORKSPACE, "tfh_train_step: workspace of %zu bytes, need %zu",
                 workspace_bytes, cv.off);
   take_act(cv, cfg, mlp, num_rays, ws);
+  take_line_copies(cv, density, appearance, ws);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mse_out, (int64_t)1, 0.0f);
   TFH_TRY(check_launch("train step: clear mse"));
   RayIn in = make_rayin(cfg, aabb, origins, directions, num_rays, jitter, base_ts, base_steps,
@@ -850,7 +907,8 @@ size_t tfh_train_step_proposal_workspace_size(const tfh_render_config* cfg, cons
   RenderWs ws;
   carve_render(cv, cfg, mlp, 3 * appearance_channels, num_rays, ws);
   cv.take<float>(num_rays * (cfg->density_samples_per_ray + 1));
-  return cv.off + prop_workspace_bytes(pc, field, num_rays) + act_bytes(cfg, mlp, num_rays);
+  return cv.off + prop_workspace_bytes(pc, field, num_rays) + act_bytes(cfg, mlp, num_rays) +
+         line_copy_bytes();
 }
 
 int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
@@ -907,6 +965,7 @@ int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
     return fail(TFH_ERR_WORKSPACE, "%s: workspace of %zu bytes, need %zu", who, workspace_bytes,
                 cv.off);
   take_act(cv, cfg, mlp, R, ws);
+  take_line_copies(cv, density, appearance, ws);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mse_out, (int64_t)1, 0.0f);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, prop_loss_out, (int64_t)1, 0.0f);
   TFH_TRY(check_launch("train step: clear losses"));

@@ -1,5 +1,7 @@
 // TensorVM.interpolate (tensor_vm.py:42-89, 140-167, 226-250), its VJP and
 // TensorVM.resize (tensor_vm.py:91-100, 183-223) as gfx950 kernels.
+#include <algorithm>
+
 #include "tfh_vm.h"
 #include "tfh_vm_kernels.h"
 
@@ -64,6 +66,42 @@ __global__ void __launch_bounds__(256) k_resize_line(const float* __restrict__ s
     const float* s = src + (size_t)m * gin * C;
     dst[e] = (1.0f - f) * s[(size_t)i0 * C + c] + f * s[(size_t)i1 * C + c];
   }
+}
+
+__global__ void __launch_bounds__(256) k_zero_copies(float* __restrict__ p, int64_t n) {
+  for (int64_t e = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4; e < n;
+       e += (int64_t)gridDim.x * 256 * 4) {
+    if (e + 4 <= n)
+      *reinterpret_cast<float4*>(p + e) = make_float4(0.f, 0.f, 0.f, 0.f);
+    else
+      for (int64_t j = e; j < n; ++j) p[j] = 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sum_copies(const float* __restrict__ copies, int ncopy,
+                                                    int64_t len, float* __restrict__ dst) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < len;
+       e += (int64_t)gridDim.x * 256) {
+    float sum = 0.0f;
+    for (int k = 0; k < ncopy; ++k) sum += copies[k * len + e];
+    dst[e] += sum;
+  }
+}
+
+// p must be 16-byte aligned (a workspace region).
+int32_t zero_copies(float* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return TFH_OK;
+  const int64_t blocks = std::min<int64_t>(ceil_div(n, 1024), 4096);
+  hipLaunchKernelGGL(k_zero_copies, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+  return check_launch("clear gradient copies");
+}
+
+int32_t sum_copies(const float* copies, int ncopy, int64_t len, float* dst, hipStream_t st) {
+  if (len <= 0) return TFH_OK;
+  const int64_t blocks = std::min<int64_t>(ceil_div(len, 256), 4096);
+  hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)blocks), dim3(256), 0, st, copies, ncopy, len,
+                     dst);
+  return check_launch("sum gradient copies");
 }
 
 }  // namespace tfh

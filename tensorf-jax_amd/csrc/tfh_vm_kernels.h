@@ -182,6 +182,11 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src sr
   }
 }
 
+// Gradient copies of a scatter (npc / nlc above): clear n floats; add the sum
+// of ncopy consecutive copies of len floats into dst (tfh_vm.hip).
+int32_t zero_copies(float* p, int64_t n, hipStream_t st);
+int32_t sum_copies(const float* copies, int ncopy, int64_t len, float* dst, hipStream_t st);
+
 template <class Src, bool ROW_MAJOR>
 inline int32_t launch_vm_scatter(const tfh_vm& vm, const Src& src, int64_t M, const float* gout,
                                  const tfh_vm_grad& grad, hipStream_t st, int npc = 1,
