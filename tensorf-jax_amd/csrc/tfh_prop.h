@@ -36,6 +36,7 @@ struct PropWs {
   // the scatter's blocks add into them round-robin, one kernel sums them
   float* gcopy = nullptr;
   int npc = 1, nlc = 1;
+  float* loss_part = nullptr;  // kLossSlots partial sums of the interval loss
 };
 
 constexpr int kNzBlock = 1024;

@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(64) k_prop_loss_bwd(
     const float* __restrict__ fedges, const float* __restrict__ fw, int nf,
     const float* __restrict__ pedges, const float* __restrict__ pw, const float* __restrict__ pe,
     const float* __restrict__ sigma, int n, float scale, float* __restrict__ dz,
-    float* __restrict__ loss_out) {
+    float* __restrict__ loss_out, int nslot) {
   extern __shared__ float sm[];
   const int64_t r = blockIdx.x;
   const LossLds L(nf, n);
@@ -240,7 +240,8 @@ __global__ void __launch_bounds__(64) k_prop_loss_bwd(
   __syncthreads();
   const float loss =
       wave_interval_loss(s_fe, fw + r * nf, nf, s_pe, s_pw, n, scale, s_g, s_G);
-  if (threadIdx.x == 0 && loss != 0.0f) unsafeAtomicAdd(loss_out, loss);
+  // ray r adds into partial sum r % nslot (adds to one address serialise)
+  if (threadIdx.x == 0 && loss != 0.0f) unsafeAtomicAdd(loss_out + r % nslot, loss);
   wave_segment_vjp(s_G, nullptr, s_pw, s_e, s_da, n);
   __syncthreads();
   for (int j = threadIdx.x; j < n; j += 64) {
@@ -358,6 +359,7 @@ size_t carve_prop(Carve& cv, const tfh_proposal_config* pc, const tfh_vm* field,
   ws.npc = (int)std::max<int64_t>(1, std::min<int64_t>(32, (16 << 20) / (pn * 4)));
   ws.nlc = (int)std::max<int64_t>(1, std::min<int64_t>(256, (2 << 20) / (ln * 4)));
   ws.gcopy = cv.take<float>(ws.npc * pn + ws.nlc * ln);
+  ws.loss_part = cv.take<float>(kLossSlots);
   return cv.off;
 }
 
@@ -491,13 +493,16 @@ int32_t prop_loss_bwd(const tfh_vm* field, const tfh_prop_mlp* mlp,
   const float scale = pc->loss_weight / (float)R;
   {
     ProfScope ps("prop_loss_bwd", st);
+    TFH_TRY(zero_copies(ws.loss_part, kLossSlots, st));
     for (int k = 0; k < ws.rounds; ++k) {
       const PropRound& rd = ws.rd[k];
       const size_t lds = (size_t)LossLds(nf, rd.n).total * sizeof(float);
       hipLaunchKernelGGL(k_prop_loss_bwd, dim3((unsigned)R), dim3(64), lds, st, fedges, fw, nf,
-                         rd.edges, rd.w, rd.e, rd.sigma, rd.n, scale, rd.dz, loss_out);
+                         rd.edges, rd.w, rd.e, rd.sigma, rd.n, scale, rd.dz, ws.loss_part,
+                         kLossSlots);
       TFH_TRY(check_launch("proposal loss bwd"));
     }
+    TFH_TRY(sum_copies(ws.loss_part, kLossSlots, 1, loss_out, st));
   }
   {
     ProfScope ps("prop_compact", st);

@@ -212,7 +212,9 @@ __global__ void __launch_bounds__(64) k_density_pass(tfh_vm dvm, RayIn in, Densi
 }
 
 // C = U * sum_j w_{i_j} c_j + E_{N-1} (render.py:233-244). With colors != null
-// also g = 2 (C - target) / (3R) and mse += |C - target|^2 / (3R).
+// also g = 2 (C - target) / (3R) and mse[r % nslot] += |C - target|^2 / (3R):
+// adds to one address serialise at the memory side, so the rays spread over
+// nslot partial sums that the caller adds up.
 __global__ void __launch_bounds__(64) k_composite_rgb(const float* __restrict__ w,
                                                       const int32_t* __restrict__ idx,
                                                       const float* __restrict__ rgb,
@@ -220,7 +222,8 @@ __global__ void __launch_bounds__(64) k_composite_rgb(const float* __restrict__ 
                                                       float* __restrict__ out,
                                                       const float* __restrict__ colors,
                                                       float* __restrict__ gout,
-                                                      float* __restrict__ mse, float inv_count) {
+                                                      float* __restrict__ mse, int nslot,
+                                                      float inv_count) {
   const int64_t r = blockIdx.x;
   const int lane = threadIdx.x;
   float acc[3] = {0.f, 0.f, 0.f};
@@ -246,7 +249,7 @@ __global__ void __launch_bounds__(64) k_composite_rgb(const float* __restrict__ 
       }
     }
   }
-  if (colors && lane == 0) unsafeAtomicAdd(mse, sq * inv_count);
+  if (colors && lane == 0) unsafeAtomicAdd(mse + r % nslot, sq * inv_count);
 }
 
 // Backward of the compositing (RGB mode) for one ray per wave, followed by
@@ -482,6 +485,7 @@ struct RenderWs {
   void* mlp_ws;
   size_t mlp_ws_bytes;
   float* act = nullptr;  // (M, 256) MLP activations kept for the backward, or null
+  float* mse_part = nullptr;  // kLossSlots partial sums of the MSE
   // copies of the density / appearance line gradients for the scatters, or null
   float* dlcopy = nullptr;
   float* alcopy = nullptr;
@@ -526,6 +530,7 @@ static size_t carve_render(Carve& cv, const tfh_render_config* cfg, const tfh_ml
   ws.gout = cv.take<float>(R * 3);
   ws.dc = cv.take<float>(M * 3);
   ws.dfeat = cv.take<float>(M * F);
+  ws.mse_part = cv.take<float>(kLossSlots);
   // one region serves the forward weight tables and, later, the backward's
   // tables + partials (the backward's need is the larger)
   ws.mlp_ws_bytes = mlp ? mlp_bwd_workspace(mlp, M, cfg->mlp_precision) : 0;
@@ -633,9 +638,13 @@ static int32_t render_forward(const tfh_vm* density, const tfh_vm* appearance, c
                          ws.mlp_ws_bytes, st, ws.act));
   }
   ProfScope ps("composite_rgb", st);
+  const bool part = colors && mse && ws.mse_part;
+  if (part) TFH_TRY(zero_copies(ws.mse_part, kLossSlots, st));
   hipLaunchKernelGGL(k_composite_rgb, dim3((unsigned)R), dim3(64), 0, st, ws.d.w, ws.d.idx, ws.rgb,
-                     ws.d.ray, N, k, out, colors, ws.gout, mse, 1.0f / (3.0f * (float)R));
-  return check_launch("render: composite");
+                     ws.d.ray, N, k, out, colors, ws.gout, part ? ws.mse_part : mse,
+                     part ? kLossSlots : 1, 1.0f / (3.0f * (float)R));
+  TFH_TRY(check_launch("render: composite"));
+  return part ? sum_copies(ws.mse_part, kLossSlots, 1, mse, st) : TFH_OK;
 }
 
 static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,

@@ -185,6 +185,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_scatter(tfh_vm vm, Src sr
 // Gradient copies of a scatter (npc / nlc above): clear n floats; add the sum
 // of ncopy consecutive copies of len floats into dst (tfh_vm.hip).
 int32_t zero_copies(float* p, int64_t n, hipStream_t st);
+// partial sums of a per-ray scalar (a loss), added up by sum_copies
+constexpr int kLossSlots = 256;
 int32_t sum_copies(const float* copies, int ncopy, int64_t len, float* dst, hipStream_t st);
 
 template <class Src, bool ROW_MAJOR>
