@@ -38,7 +38,10 @@ def stage_model(stage, R, N, k, cd, ca, precision, n_params, prop=None):
     """Algorithmic work of one launch of a stage: (bound, amount, unit, peak).
     Bytes: VM taps 72*C per sample (3 modes x 6 taps x C x 4 B) plus the
     streamed per-sample I/O; FLOPs: 79,712 per feature-MLP row forward
-    (nf=nv=2) and 11,392 per proposal-MLP row (2 (24*64 + 64*64 + 64)).
+    (nf=nv=2) and 11,392 per proposal-MLP row (2 (24*64 + 64*64 + 64)). The
+    training step keeps the MLP's hidden activations, so its backward does
+    the input and weight gradients of every layer (2 x 79,712 per row) plus
+    the recompute of layer 0 (2 x 144 x 27).
     prop: (samples per ray over the proposal rounds, proposal channels)."""
     M = R * k
     mlp_row = 2 * (144 * 27 + 150 * 128 + 128 * 128 + 128 * 3)
@@ -50,7 +53,7 @@ def stage_model(stage, R, N, k, cd, ca, precision, n_params, prop=None):
     if stage == "mlp_fwd_kernel":
         return "mfma", M * mlp_row, "TFLOP/s", mfma_peak
     if stage == "mlp_bwd":
-        return "mfma", M * 3 * mlp_row, "TFLOP/s", mfma_peak
+        return "mfma", M * (2 * mlp_row + 2 * 144 * 27), "TFLOP/s", mfma_peak
     if stage == "composite_bwd_density_scatter":
         return "hbm", R * N * (2 * 72 * cd + 36), "GB/s", PEAK_HBM_GBS
     if stage == "appearance_scatter":
