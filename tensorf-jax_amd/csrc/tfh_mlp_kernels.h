@@ -1,10 +1,12 @@
-// FeatureMlp kernels on gfx950 MFMA, shared by three translation units:
+// FeatureMlp kernels on gfx950 MFMA, shared by four translation units:
 // tfh_mlp.hip (forward, backward of the generic shapes), tfh_mlp_bwd_std.hip
-// (backward of the standard shape) and tfh_mlp_film.hip (the same with a FiLM
-// camera embedding). The standard backward sits at the 256-VGPR limit of two
-// waves per SIMD, and the compiler allocates its registers worse when other
-// instantiations share its translation unit: 17/31 spilled VGPRs (BF16/FP32)
-// on its own, 26/43 next to the generic kernels, 39/67 next to the FiLM one.
+// (backward of the standard shape), tfh_mlp_bwd_act.hip (the same reading the
+// activations the forward kept) and tfh_mlp_film.hip (with a FiLM camera
+// embedding). The standard backward sits at the 256-VGPR limit of two waves
+// per SIMD, and the compiler allocates its registers worse when other
+// instantiations share its translation unit (once 17/31 spilled VGPRs
+// (BF16/FP32) on its own against 26/43 next to the generic kernels and 39/67
+// next to the FiLM one).
 //
 // FeatureMlp (networks.py:38-121).
 //
@@ -25,7 +27,9 @@
 //         rounded to BF16 when loaded as operands.
 // Weights are packed once per call into zero-padded [out][in] (forward) and
 // [in][out] (input-gradient) tables of the operand precision, L2-resident.
-// The backward recomputes the forward of its tile and accumulates the weight
+// The backward recomputes the forward of its tile (only layer 0 and the
+// encoding when the forward kernel kept the hidden activations, the fused
+// training steps' case) and accumulates the weight
 // gradients in registers of their owning wave (standard shape) or in a
 // per-workgroup partial buffer (other shapes, plain read-modify-write, each
 // partial tile owned by one wave), reduced across workgroups by k_mlp_reduce.
