@@ -111,11 +111,12 @@ size_t mlp_fwd_workspace(const tfh_mlp* p, int32_t precision) {
   return pack_bytes(make_dims(p, bf), bf);
 }
 
-// The standard shape without a camera embedding keeps H1 / H2 for the
-// backward when the caller provides the room (the fused training steps).
+// The standard shape keeps relu(z1) and relu(z2) (before any camera
+// embedding) for the backward when the caller provides the room (the fused
+// training steps).
 bool mlp_keeps_activations(const tfh_mlp* p, int32_t precision) {
   const bool bf = precision == TFH_PREC_BF16;
-  return p && !p->embed && is_std_dims(make_dims(p, bf), bf);
+  return p && is_std_dims(make_dims(p, bf), bf);
 }
 
 int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
@@ -126,7 +127,7 @@ int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   Carve cv(ws, ws_bytes);
   MlpPack pk;
   TFH_TRY(build_pack(p, d, bf, cv, pk, st));
-  const size_t lds = fwd_lds_floats<kFwdTM>(d) * sizeof(float);
+  const size_t lds = fwd_lds_floats<kFwdTM>(d, act != nullptr) * sizeof(float);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp fwd: tile needs %zu bytes of LDS", lds);
   const int64_t blocks = ceil_div(M, kFwdTM);
   ProfScope ps("mlp_fwd_kernel", st);
@@ -196,7 +197,10 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
                        nullptr);                                                              \
   } while (0)
     const bool film = d.ncam > 0;
-    if (sd && act && !film)
+    if (sd && act && film)
+      launch_mlp_bwd_std_film_act(bf, (unsigned)nblk, lds, st, d, to_w(p), pk, gp, feats, dirs,
+                                  cams, grgb, M, rpr, gfeat, partials, g->embed, act, rgb);
+    else if (sd && act && !film)
       launch_mlp_bwd_std_act(bf, (unsigned)nblk, lds, st, d, to_w(p), pk, gp, feats, dirs, grgb, M,
                              rpr, gfeat, partials, act, rgb);
     else if (sd && film)

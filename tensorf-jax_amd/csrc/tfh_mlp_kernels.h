@@ -213,13 +213,15 @@ __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
 // 3 colours). Each source is a buffer resource over the tile's valid rows, so
 // an offset is 32 bits and rows past M read 0 with no branch; every load is
 // issued before the first LDS store and the tile pays one memory latency.
-// Requires F % 4 == 0 and 0 < rpr < 2^31.
-template <int TM, int NT, bool ACT>
+// With CAMS also each row's camera index (Cs, clamped to [0, ncam), from the
+// per-ray cams). Requires F % 4 == 0 and 0 < rpr < 2^31.
+template <int TM, int NT, bool ACT, bool CAMS = false>
 __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats,
                                const float* __restrict__ dirs, const float* __restrict__ act,
                                const float* __restrict__ rgb, const float* __restrict__ grgb,
                                int64_t row0, int64_t M, int64_t rpr, float* Fs, float* Ds,
-                               float* H1, float* H2, float* Z3) {
+                               float* H1, float* H2, float* Z3,
+                               const uint32_t* __restrict__ cams = nullptr, int* Cs = nullptr) {
   typedef int i32x4 __attribute__((ext_vector_type(4)));
   constexpr int kA4 = ACT ? TM * 64 / NT : 1;  // float4 of activations per thread
   static_assert(TM * 64 % NT == 0 && TM * 3 <= NT, "tile shape");
@@ -237,6 +239,8 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
   const auto rs = buffer_rsrc(ACT ? rgb + row0 * 3 : feats, ACT ? nv * 12 : 0);
   const auto rg = buffer_rsrc(ACT ? grgb + row0 * 3 : feats, ACT ? nv * 12 : 0);
   const auto rd = buffer_rsrc(dirs + ray0 * 3, nray * 12);
+  const auto rc = buffer_rsrc(CAMS ? static_cast<const void*>(cams + ray0) : feats,
+                              CAMS ? nray * 4 : 0);
   const int fp4 = d.FP / 4, f4 = d.F / 4, n4 = TM * fp4;
   const int kF4 = (n4 + NT - 1) / NT;  // constant for the specialised shape
   const int t = threadIdx.x;
@@ -264,6 +268,11 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
   const unsigned rel = (rem0 + (unsigned)i3) / (unsigned)rpr;  // ray of row i3 after ray0
   const float dv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
       rd, (own3 && i3 < nv) ? (int)(rel * 3 + c3) * 4 : kOOB, 0, 0));
+  int cam = 0;
+  if constexpr (CAMS) {
+    const unsigned relc = (rem0 + (unsigned)t) / (unsigned)rpr;  // ray of row t after ray0
+    cam = __builtin_amdgcn_raw_buffer_load_b32(rc, (t < nv) ? (int)relc * 4 : kOOB, 0, 0);
+  }
   // keeps the loads here: a load whose only use is a predicated store would
   // otherwise be sunk into that branch, after the waits of the others
   asm volatile("" ::: "memory");
@@ -275,6 +284,9 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
       *reinterpret_cast<float4*>(Fs + i * d.ldF + 4 * c4) = __builtin_bit_cast(float4, vf[k]);
   }
   if (own3) Ds[i3 * 3 + c3] = dv;
+  if constexpr (CAMS) {
+    if (t < TM) Cs[t] = (cam < 0 || cam >= d.ncam) ? 0 : cam;
+  }
   if constexpr (ACT) {
 #pragma unroll
     for (int k = 0; k < kA4; ++k) {
@@ -429,9 +441,11 @@ __device__ void forward_out(const MlpDims& d, const MlpW& w, const Operands<BF>&
 
 // ---- forward kernel -----------------------------------------------------------
 template <int TM>
-inline size_t fwd_lds_floats(const MlpDims& d) {
-  // the layer-3 output reuses region A (H1 is dead by then)
-  return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + 4);
+inline size_t fwd_lds_floats(const MlpDims& d, bool keep = false) {
+  // the layer-3 output reuses region A (H1 is dead by then); keep: a region
+  // for relu(z2) before the camera embedding (kept activations with FiLM)
+  return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + 4 +
+                       (keep && d.ncam > 0 ? d.ldU : 0));
 }
 
 // act (optional, no camera embedding): each row's relu(z1) and layer-3 input,
@@ -448,28 +462,36 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   extern __shared__ float smem[];
   // regions: A = Fs, later H1, last R3 (the layer-3 output); B = X, later
   // H2; small: Ds, Cs. 40 KB per workgroup for the FP32 standard shape: four
-  // workgroups share a CU.
+  // workgroups share a CU. With a camera embedding and act, one more region
+  // keeps relu(z2) before the embedding (fwd_lds_floats(d, true)).
   float* regA = smem;
   float* regB = regA + TM * std::max(d.ldF, d.ldU);
   float* Ds = regB + TM * std::max(d.ldE, d.ldU);
   int* Cs = reinterpret_cast<int*>(Ds + TM * 4);
   float* R3 = regA;
+  float* H2keep = act && d.ncam > 0 ? reinterpret_cast<float*>(Cs + TM * 4) : regB;
   const Operands<BF> op{d, w, pk};
   const int64_t row0 = (int64_t)blockIdx.x * TM;
   if (STD && d.ncam == 0)
     load_tile_rows<TM, NW * 64, false>(d, feats, dirs, nullptr, nullptr, nullptr, row0, M, rpr,
-                                       regA, Ds, nullptr, nullptr, nullptr);
+                                       regA, Ds, nullptr, nullptr, nullptr, nullptr, nullptr);
+  else if (STD)
+    load_tile_rows<TM, NW * 64, false, true>(d, feats, dirs, nullptr, nullptr, nullptr, row0, M,
+                                             rpr, regA, Ds, nullptr, nullptr, nullptr, cams, Cs);
   else
     load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, regA, Ds, Cs);
   __syncthreads();
   // layer 1 reads X (regB) and writes H1 over Fs (regA, dead after layer 0);
   // layer 2 reads H1 and writes H2 over X (dead after layer 1)
-  forward_hidden<TM, NW, BF, TM / 16>(d, w, op, regA, regB, nullptr, regA, regB, regB, Ds, Cs);
-  if (act && d.ncam == 0) {
+  forward_hidden<TM, NW, BF, TM / 16>(d, w, op, regA, regB, nullptr, regA, H2keep, regB, Ds, Cs);
+  if (act) {
+    // per row: relu(z1), then relu(z2) before the embedding (the layer-3 input
+    // without one)
     for (int e = threadIdx.x; e < TM * 64; e += blockDim.x) {
       const int i = e >> 6, c4 = e & 63;
       if (row0 + i < M) {
-        const float* src = c4 < 32 ? regA + i * d.ldU + 4 * c4 : regB + i * d.ldU + 4 * (c4 - 32);
+        const float* src =
+            c4 < 32 ? regA + i * d.ldU + 4 * c4 : H2keep + i * d.ldU + 4 * (c4 - 32);
         *reinterpret_cast<float4*>(act + (row0 + i) * 256 + 4 * c4) =
             *reinterpret_cast<const float4*>(src);
       }
@@ -666,8 +688,9 @@ constexpr int kCnt3 = (8 + kBwdNW - 1) / kBwdNW, kLSlot3 = 0;
 constexpr int kCnt0 = (20 + kBwdNW - 1) / kBwdNW, kLSlot0 = kLSlot3 + kCnt3;
 constexpr int kLdsSlots = kLSlot0 + kCnt0;
 
-// ACT: H1, H2 and the output come from the forward kernel's act / rgb (no
-// camera embedding): only layer 0 and the encoding are recomputed.
+// ACT: H1, H2 (or, with FiLM, relu(z2) before the embedding) and the output
+// come from the forward kernel's act / rgb: only layer 0, the encoding (and
+// the embedding's scale and shift) are recomputed.
 template <int TM, int NW, bool BF, int NT, bool STD, bool FILM, bool ACT = false>
 __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpPack pk, GradPack gp_arg,
                                                      const float* __restrict__ feats,
@@ -731,8 +754,9 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     const int64_t row0 = tile * TM;
     __syncthreads();
     if constexpr (ACT) {
-      load_tile_rows<TM, NW * 64, true>(d, feats, dirs, act, rgb_in, grgb, row0, M, rpr, Fs, Ds,
-                                        H1, H2, Z3);
+      // with an embedding the kept second half is relu(z2) before it (H2p)
+      load_tile_rows<TM, NW * 64, true, FILM>(d, feats, dirs, act, rgb_in, grgb, row0, M, rpr, Fs,
+                                              Ds, H1, FILM ? H2p : H2, Z3, cams, Cs);
     } else {
       load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
     }
@@ -756,6 +780,21 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     }
     forward_hidden<TM, NW, BF, BRTI, !ACT>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
     if constexpr (!ACT) forward_out<TM, NW, BF>(d, w, op, H2, Z3);
+    if constexpr (ACT && FILM) {
+      // the layer-3 input from the kept relu(z2): the embedding's scale and
+      // shift on the second half of the columns (forward_hidden's layer 2)
+      for (int e = threadIdx.x; e < TM * kU; e += blockDim.x) {
+        const int i = e / kU, c = e - i * kU;
+        const float h = H2p[i * d.ldU + c];
+        float out = h;
+        if (d.ncam > 0 && c >= kU / 2) {
+          const float* emb = w.embed + (size_t)Cs[i] * kU;
+          out = emb[c - kU / 2] * h + emb[c];
+        }
+        H2[i * d.ldU + c] = out;
+      }
+      __syncthreads();
+    }
     // dZ3 = grgb * s * (1 - s), zero outside the 3 colours and the valid rows
     // (loaded with the tile when the activations are kept)
     if constexpr (!ACT) {
@@ -995,6 +1034,12 @@ void launch_mlp_bwd_std_act(bool bf, unsigned nblk, size_t lds, hipStream_t st,
                             const GradPack& gp, const float* feats, const float* dirs,
                             const float* grgb, int64_t M, int64_t rpr, float* gfeat,
                             float* partials, const float* act, const float* rgb);
+void launch_mlp_bwd_std_film_act(bool bf, unsigned nblk, size_t lds, hipStream_t st,
+                                 const MlpDims& d, const MlpW& w, const MlpPack& pk,
+                                 const GradPack& gp, const float* feats, const float* dirs,
+                                 const uint32_t* cams, const float* grgb, int64_t M, int64_t rpr,
+                                 float* gfeat, float* partials, float* gembed, const float* act,
+                                 const float* rgb);
 void launch_mlp_bwd_std_film(bool bf, unsigned nblk, size_t lds, hipStream_t st,
                              const MlpDims& d, const MlpW& w, const MlpPack& pk,
                              const GradPack& gp, const float* feats, const float* dirs,

@@ -176,7 +176,10 @@ def test_train_step_matches_oracle():
     assert abs(mse - ologs["mse"]) <= 1e-5 * ologs["mse"]
     lp = st.learnable_params
     dv, dm = lp.density_tensor.to_reference_layout()
+    av, am = lp.appearance_tensor.to_reference_layout()
+    # the factor lines go through the training step's gradient copies
     got = {"density_vector": dv.cpu().numpy(), "density_matrix": dm.cpu().numpy(),
+           "appearance_vector": av.cpu().numpy(), "appearance_matrix": am.cpu().numpy(),
            "w1": lp.appearance_mlp_params["params"]["Dense_1"]["kernel"].cpu().numpy()}
     # Adam's first step moves every parameter with a non-zero gradient by ~lr:
     # compare the update itself, relative to lr
@@ -184,7 +187,7 @@ def test_train_step_matches_oracle():
         upd_gpu = got[key] - p[key]
         upd_ref = newp[key] - p[key]
         e = report(f"update {key}", upd_gpu, upd_ref)
-        lr = 0.02 if "density" in key else 1e-3
+        lr = 1e-3 if key.startswith("w") else 0.02
         # sign flips of tiny gradients move a parameter by up to 2 lr: count them
         frac_bad = np.mean(e > 1e-3 * lr)
         log(f"  fraction of {key} updates off by >1e-3 lr: {frac_bad:.2e}")

@@ -165,20 +165,22 @@ def test_proposal64_matches_uniform512_at_a_fraction_of_the_samples():
     assert sec["proposal-64"] * 3.0 <= sec["uniform-512"]
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16"])
-def test_train_step_kept_activations_match_recompute(monkeypatch, precision):
+@pytest.mark.parametrize("precision,film", [("fp32", False), ("bf16", False), ("fp32", True),
+                                            ("bf16", True)])
+def test_train_step_kept_activations_match_recompute(monkeypatch, precision, film):
     """The fused training step keeps the MLP's hidden activations between its
     forward and backward when the workspace has the room
     (tfh_train_step_workspace_size) and recomputes them with a render-sized
     workspace; both give the same step: the same MSE bit for bit, and the same
     Adam update except where the order of the scatter atomics tips a
     near-zero gradient (at most 1e-3 of the parameters off by > 1e-3 lr)."""
-    log(f"case: train step, kept activations vs recompute, {precision}")
+    log(f"case: train step, kept activations vs recompute, {precision}, FiLM {film}")
     from tensorf_amd import _lib, training
     _, c2ws, data = _small_dataset(views=4, res=32)
     out = []
     for keep in (True, False):
-        cfg = dataclasses.replace(_config(g=24, n=64), mlp_precision=precision)
+        cfg = dataclasses.replace(_config(g=24, n=64), mlp_precision=precision,
+                                  camera_embeddings=film)
         st = training.TrainState.initialize(cfg, 24, prng_key=5, num_cameras=4)
         p0 = st.flat.params.clone()
         if not keep:
