@@ -127,7 +127,7 @@ int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   Carve cv(ws, ws_bytes);
   MlpPack pk;
   TFH_TRY(build_pack(p, d, bf, cv, pk, st));
-  const size_t lds = fwd_lds_floats<kFwdTM>(d, act != nullptr) * sizeof(float);
+  const size_t lds = fwd_lds_floats<kFwdTM>(d) * sizeof(float);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp fwd: tile needs %zu bytes of LDS", lds);
   const int64_t blocks = ceil_div(M, kFwdTM);
   ProfScope ps("mlp_fwd_kernel", st);

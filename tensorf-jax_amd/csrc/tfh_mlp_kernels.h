@@ -342,9 +342,12 @@ __device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
 // spreads a layer over more waves.
 // L12 == false stops after the encoding (H1 and H2 come from elsewhere).
 template <int TM, int NW, bool BF, int RTI, bool L12 = true>
+// h2g (optional, rows < nvalid): relu(z2) also stored to global memory, row
+// stride 256 floats (the forward's kept activations with an embedding).
 __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
                                const float* Fs, float* X, float* H0, float* H1, float* H2p,
-                               float* H2, const float* Ds, const int* Cs) {
+                               float* H2, const float* Ds, const int* Cs,
+                               float* __restrict__ h2g = nullptr, int nvalid = 0) {
   constexpr int RG = TM / 16 / RTI;  // row groups
   // B prefetch depth (K per pass): the whole strip with few waves, 64 when the
   // workgroup's waves share the register file (16 waves)
@@ -408,6 +411,7 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
           out = emb[col - kU / 2] * h + emb[col];
         }
         if (H2p != H2) H2p[row * d.ldU + col] = h;
+        if (h2g && row < nvalid) h2g[row * 256 + col] = h;
         H2[row * d.ldU + col] = out;
       }
   }
@@ -441,11 +445,9 @@ __device__ void forward_out(const MlpDims& d, const MlpW& w, const Operands<BF>&
 
 // ---- forward kernel -----------------------------------------------------------
 template <int TM>
-inline size_t fwd_lds_floats(const MlpDims& d, bool keep = false) {
-  // the layer-3 output reuses region A (H1 is dead by then); keep: a region
-  // for relu(z2) before the camera embedding (kept activations with FiLM)
-  return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + 4 +
-                       (keep && d.ncam > 0 ? d.ldU : 0));
+inline size_t fwd_lds_floats(const MlpDims& d) {
+  // the layer-3 output reuses region A (H1 is dead by then)
+  return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + 4);
 }
 
 // act (optional, no camera embedding): each row's relu(z1) and layer-3 input,
@@ -462,14 +464,12 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   extern __shared__ float smem[];
   // regions: A = Fs, later H1, last R3 (the layer-3 output); B = X, later
   // H2; small: Ds, Cs. 40 KB per workgroup for the FP32 standard shape: four
-  // workgroups share a CU. With a camera embedding and act, one more region
-  // keeps relu(z2) before the embedding (fwd_lds_floats(d, true)).
+  // workgroups share a CU.
   float* regA = smem;
   float* regB = regA + TM * std::max(d.ldF, d.ldU);
   float* Ds = regB + TM * std::max(d.ldE, d.ldU);
   int* Cs = reinterpret_cast<int*>(Ds + TM * 4);
   float* R3 = regA;
-  float* H2keep = act && d.ncam > 0 ? reinterpret_cast<float*>(Cs + TM * 4) : regB;
   const Operands<BF> op{d, w, pk};
   const int64_t row0 = (int64_t)blockIdx.x * TM;
   if (STD && d.ncam == 0)
@@ -483,15 +483,18 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   __syncthreads();
   // layer 1 reads X (regB) and writes H1 over Fs (regA, dead after layer 0);
   // layer 2 reads H1 and writes H2 over X (dead after layer 1)
-  forward_hidden<TM, NW, BF, TM / 16>(d, w, op, regA, regB, nullptr, regA, H2keep, regB, Ds, Cs);
+  // kept activations per row: relu(z1), then relu(z2) before the embedding
+  // (the layer-3 input without one); with an embedding layer 2 stores the
+  // second half itself
+  const bool film_keep = act && d.ncam > 0;
+  forward_hidden<TM, NW, BF, TM / 16>(d, w, op, regA, regB, nullptr, regA, regB, regB, Ds, Cs,
+                                      film_keep ? act + row0 * 256 + 128 : nullptr,
+                                      (int)min((int64_t)TM, M - row0));
   if (act) {
-    // per row: relu(z1), then relu(z2) before the embedding (the layer-3 input
-    // without one)
     for (int e = threadIdx.x; e < TM * 64; e += blockDim.x) {
       const int i = e >> 6, c4 = e & 63;
-      if (row0 + i < M) {
-        const float* src =
-            c4 < 32 ? regA + i * d.ldU + 4 * c4 : H2keep + i * d.ldU + 4 * (c4 - 32);
+      if (row0 + i < M && (c4 < 32 || !film_keep)) {
+        const float* src = c4 < 32 ? regA + i * d.ldU + 4 * c4 : regB + i * d.ldU + 4 * (c4 - 32);
         *reinterpret_cast<float4*>(act + (row0 + i) * 256 + 4 * c4) =
             *reinterpret_cast<const float4*>(src);
       }
