@@ -211,10 +211,13 @@ int32_t tfh_feature_mlp_bwd(const tfh_mlp* mlp, const float* features, const flo
 size_t tfh_render_workspace_size(const tfh_render_config* cfg, const tfh_mlp* mlp,
                                  int64_t appearance_channels, int64_t num_rays);
 /* Workspace that lets tfh_train_step keep the feature MLP's hidden
- * activations (1 KB per shaded sample, standard 48-channel shape without a
- * camera embedding) between its forward and backward instead of recomputing
- * layers 1-3; with a tfh_render_workspace_size workspace it recomputes them.
- * No counterpart in the reference (XLA keeps activations by itself). */
+ * activations (1 KB per shaded sample, standard 48-channel shape, with or
+ * without a camera embedding) between its forward and backward instead of
+ * recomputing layers 1-3, plus 12 MB for copies of the factor-line gradients
+ * that spread the scatters' atomics; with a tfh_render_workspace_size
+ * workspace it recomputes the activations and adds straight into the
+ * gradients. No counterpart in the reference (XLA keeps activations by
+ * itself). */
 size_t tfh_train_step_workspace_size(const tfh_render_config* cfg, const tfh_mlp* mlp,
                                      int64_t appearance_channels, int64_t num_rays);
 /* render_rays (render.py:105-279): rays -> out (R,3) for RGB, (R) for the
