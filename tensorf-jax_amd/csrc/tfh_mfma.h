@@ -72,6 +72,16 @@ struct PackedB {
   }
 };
 
+// A BF16 table copied into LDS (stride ld elements, 16-byte aligned rows) as a
+// B operand: B(k, j) = T[j][k].
+struct LdsB {
+  const __bf16* t;
+  int ld;
+  __device__ __forceinline__ bf16x8 load8(int k, int j) const {
+    return *reinterpret_cast<const bf16x8*>(t + j * ld + k);
+  }
+};
+
 // FP32 table T[j][k] (stride ld, zero padded) as a B operand: B(k, j) = T[j][k];
 // four consecutive k of one column are one 16-byte load.
 struct PackedB32 {
@@ -157,9 +167,9 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
   }
 }
 
-template <int NR, int KMAX>
-__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB b,
-                                          int col0, f32x4 (&acc)[NR]) {
+template <int NR, int KMAX, class B16>
+__device__ __forceinline__ void mma_strip_b16(const float* __restrict__ A, int lda, int K, B16 b,
+                                              int col0, f32x4 (&acc)[NR]) {
   const int lane = lane_id();
   const int r = lane & 15, kq = lane >> 4;
   constexpr int NS = (KMAX + 31) / 32;
@@ -181,6 +191,18 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
       }
     }
   }
+}
+
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  mma_strip_b16<NR, KMAX>(A, lda, K, b, col0, acc);
+}
+
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, LdsB b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  mma_strip_b16<NR, KMAX>(A, lda, K, b, col0, acc);
 }
 
 // Weight-gradient tile: acc(i, j) = sum_k X[k][row0 + i] * Y[k][col0 + j],

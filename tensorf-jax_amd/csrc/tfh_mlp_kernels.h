@@ -690,6 +690,12 @@ constexpr int kSlots = kSlot1 + kCnt1;
 constexpr int kCnt3 = (8 + kBwdNW - 1) / kBwdNW, kLSlot3 = 0;
 constexpr int kCnt0 = (20 + kBwdNW - 1) / kBwdNW, kLSlot0 = kLSlot3 + kCnt3;
 constexpr int kLdsSlots = kLSlot0 + kCnt0;
+// BF16 standard shape without an embedding: the rows of the dX strip's B
+// table (g1 rows below fe1 = 138, 9 column tiles) are copied into LDS once per
+// workgroup behind the weight-gradient slots, rows padded to 272 bytes (39 KB;
+// 154 KB in all), so that strip reads LDS instead of L2.
+constexpr int kG1Rows = 144, kG1Ld = 136;
+constexpr size_t kG1LdsBytes = (size_t)kG1Rows * kG1Ld * 2;
 
 // ACT: H1, H2 (or, with FiLM, relu(z2) before the embedding) and the output
 // come from the forward kernel's act / rgb: only layer 0, the encoding (and
@@ -751,6 +757,16 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     for (int64_t e = threadIdx.x; e < gp.total; e += blockDim.x) part[e] = 0.0f;
   else
     for (int e = threadIdx.x; e < kLdsSlots * NW * 256; e += blockDim.x) Wl[e] = 0.0f;
+  constexpr bool kG1Lds = BF && STD && !FILM && NT > 0;
+  __bf16* G1s = reinterpret_cast<__bf16*>(Wl + kLdsSlots * NW * 256);
+  if constexpr (kG1Lds) {
+    // the tile loop starts with a barrier
+    for (int e = threadIdx.x; e < kG1Rows * (kU / 8); e += blockDim.x) {
+      const int j = e / (kU / 8), c8 = e - j * (kU / 8);
+      *reinterpret_cast<uint4*>(G1s + j * kG1Ld + 8 * c8) =
+          *reinterpret_cast<const uint4*>(pk.g1 + j * kU + 8 * c8);
+    }
+  }
 
   const int64_t ntiles = (M + TM - 1) / TM;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -911,7 +927,10 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
         const int c0 = 16 * (it / RT), r0 = 16 * (it % RT);
         f32x4 acc[1];
         zero_acc(acc);
-        mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
+        if constexpr (kG1Lds)
+          mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, LdsB{G1s, kG1Ld}, c0, acc);
+        else
+          mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
         const int col = c0 + (lane & 15);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
