@@ -72,6 +72,16 @@ struct PackedB {
   }
 };
 
+// An FP32 table copied into LDS (stride ld floats, 16-byte aligned rows) as a
+// B operand: B(k, j) = T[j][k].
+struct LdsB32 {
+  const float* t;
+  int ld;
+  __device__ __forceinline__ float4 load4(int k, int j) const {
+    return *reinterpret_cast<const float4*>(t + j * ld + k);
+  }
+};
+
 // A BF16 table copied into LDS (stride ld elements, 16-byte aligned rows) as a
 // B operand: B(k, j) = T[j][k].
 struct LdsB {
@@ -102,9 +112,9 @@ struct PackedB32 {
 // four B operands one 16-byte buffer load. The sum over k is unchanged; only
 // the order of the FP32 accumulation differs from a k-ascending chain.
 // Requires K % 16 == 0, lda % 4 == 0 and a 16-byte aligned A.
-template <int NR, int KMAX>
-__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB32 b,
-                                          int col0, f32x4 (&acc)[NR]) {
+template <int NR, int KMAX, class B4>
+__device__ __forceinline__ void mma_strip_f32(const float* __restrict__ A, int lda, int K, B4 b,
+                                              int col0, f32x4 (&acc)[NR]) {
   const int lane = lane_id();
   const int r = lane & 15, kq = lane >> 4;
   constexpr int NS = KMAX / 16;
@@ -138,6 +148,18 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
       __builtin_amdgcn_sched_barrier(0);
     }
   }
+}
+
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB32 b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  mma_strip_f32<NR, KMAX>(A, lda, K, b, col0, acc);
+}
+
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, LdsB32 b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  mma_strip_f32<NR, KMAX>(A, lda, K, b, col0, acc);
 }
 
 // acc[rt] += A(rows 16*rt.., K) * B(K, cols col0..col0+15), rt < NR; A from

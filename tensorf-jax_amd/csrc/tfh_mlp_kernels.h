@@ -696,6 +696,10 @@ constexpr int kLdsSlots = kLSlot0 + kCnt0;
 // 154 KB in all), so that strip reads LDS instead of L2.
 constexpr int kG1Rows = 144, kG1Ld = 136;
 constexpr size_t kG1LdsBytes = (size_t)kG1Rows * kG1Ld * 2;
+// FP32: the first 80 rows (5 of the 9 column tiles, rows padded to 528 bytes,
+// 41 KB; 153 KB in all); the other column tiles read L2.
+constexpr int kG1RowsF = 80, kG1LdF = 132;
+constexpr size_t kG1LdsBytesF = (size_t)kG1RowsF * kG1LdF * 4;
 
 // ACT: H1, H2 (or, with FiLM, relu(z2) before the embedding) and the output
 // come from the forward kernel's act / rgb: only layer 0, the encoding (and
@@ -758,13 +762,22 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   else
     for (int e = threadIdx.x; e < kLdsSlots * NW * 256; e += blockDim.x) Wl[e] = 0.0f;
   constexpr bool kG1Lds = BF && STD && !FILM && NT > 0;
+  constexpr bool kG1LdsF = !BF && STD && !FILM && NT > 0;
   __bf16* G1s = reinterpret_cast<__bf16*>(Wl + kLdsSlots * NW * 256);
+  float* G1f = Wl + kLdsSlots * NW * 256;
   if constexpr (kG1Lds) {
     // the tile loop starts with a barrier
     for (int e = threadIdx.x; e < kG1Rows * (kU / 8); e += blockDim.x) {
       const int j = e / (kU / 8), c8 = e - j * (kU / 8);
       *reinterpret_cast<uint4*>(G1s + j * kG1Ld + 8 * c8) =
           *reinterpret_cast<const uint4*>(pk.g1 + j * kU + 8 * c8);
+    }
+  }
+  if constexpr (kG1LdsF) {
+    for (int e = threadIdx.x; e < kG1RowsF * (kU / 4); e += blockDim.x) {
+      const int j = e / (kU / 4), c4 = e - j * (kU / 4);
+      *reinterpret_cast<float4*>(G1f + j * kG1LdF + 4 * c4) =
+          *reinterpret_cast<const float4*>(pk.fg1 + j * kU + 4 * c4);
     }
   }
 
@@ -927,10 +940,16 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
         const int c0 = 16 * (it / RT), r0 = 16 * (it % RT);
         f32x4 acc[1];
         zero_acc(acc);
-        if constexpr (kG1Lds)
+        if constexpr (kG1Lds) {
           mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, LdsB{G1s, kG1Ld}, c0, acc);
-        else
+        } else if constexpr (kG1LdsF) {
+          if (c0 < kG1RowsF)  // wave-uniform
+            mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, LdsB32{G1f, kG1LdF}, c0, acc);
+          else
+            mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
+        } else {
           mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
+        }
         const int col = c0 + (lane & 15);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
