@@ -82,6 +82,16 @@ struct LdsB32 {
   }
 };
 
+// An FP32 table behind a generic pointer (LDS or global memory; flat loads)
+// as a B operand: B(k, j) = T[j][k].
+struct FlatB32 {
+  const float* t;
+  int ld;
+  __device__ __forceinline__ float4 load4(int k, int j) const {
+    return *reinterpret_cast<const float4*>(t + j * ld + k);
+  }
+};
+
 // A BF16 table copied into LDS (stride ld elements, 16-byte aligned rows) as a
 // B operand: B(k, j) = T[j][k].
 struct LdsB {
@@ -158,6 +168,12 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
 
 template <int NR, int KMAX>
 __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, LdsB32 b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  mma_strip_f32<NR, KMAX>(A, lda, K, b, col0, acc);
+}
+
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, FlatB32 b,
                                           int col0, f32x4 (&acc)[NR]) {
   mma_strip_f32<NR, KMAX>(A, lda, K, b, col0, acc);
 }

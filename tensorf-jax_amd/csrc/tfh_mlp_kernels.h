@@ -943,10 +943,11 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
         if constexpr (kG1Lds) {
           mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, LdsB{G1s, kG1Ld}, c0, acc);
         } else if constexpr (kG1LdsF) {
-          if (c0 < kG1RowsF)  // wave-uniform
-            mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, LdsB32{G1f, kG1LdF}, c0, acc);
-          else
-            mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
+          // one code path over a generic pointer: LDS rows or global rows
+          // (two strip variants in a branch cost the kernel 7 more spills)
+          const bool lds = c0 < kG1RowsF;  // wave-uniform
+          const FlatB32 b{lds ? static_cast<const float*>(G1f) : pk.fg1, lds ? kG1LdF : kU};
+          mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, b, c0, acc);
         } else {
           mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
         }
