@@ -214,15 +214,10 @@ size_t tfh_render_workspace_size(const tfh_render_config* cfg, const tfh_mlp* ml
  * activations (1 KB per shaded sample, standard 48-channel shape, with or
  * without a camera embedding) between its forward and backward instead of
  * recomputing layers 1-3, plus 12 MB for copies of the factor-line gradients
- * that spread the scatters' atomics, plus 4 bytes per density sample that
- * hand dL/dsigma to the density scatter, which then runs on a library-owned
- * second stream (default priority, one per device, created on first use and
- * joined back into the caller's stream before the call's later work) beside
- * the feature-MLP backward. With a tfh_render_workspace_size workspace it
- * recomputes the activations, adds straight into the gradients and keeps
- * every kernel on the caller's stream, as it also does while stage timing
- * (tfh_profile_enable) is on. No counterpart in the reference (XLA keeps
- * activations by itself). */
+ * that spread the scatters' atomics; with a tfh_render_workspace_size
+ * workspace it recomputes the activations and adds straight into the
+ * gradients. No counterpart in the reference (XLA keeps activations by
+ * itself). */
 size_t tfh_train_step_workspace_size(const tfh_render_config* cfg, const tfh_mlp* mlp,
                                      int64_t appearance_channels, int64_t num_rays);
 /* render_rays (render.py:105-279): rays -> out (R,3) for RGB, (R) for the

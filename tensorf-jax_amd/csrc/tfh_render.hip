@@ -15,7 +15,6 @@
 //   mlp_bwd          recompute + backward of the MLP -> dL/dfeatures
 //   k_vm_scatter     appearance factor gradients
 #include <algorithm>
-#include <mutex>
 
 #include "tfh_prop.h"
 
@@ -269,124 +268,110 @@ struct BwdIn {
 };
 
 // One workgroup of kCompBwdWaves waves per ray: wave 0 runs the scans, every
-// wave then scatters 4 samples at a time. PHASE selects the part: kCompBoth
-// runs both with dL/ds_i in LDS; kCompScan stops after the scans and stores
-// dL/ds_i to ds; kCompScatter only scatters dL/ds_i read from ds, so that it
-// can run on a second stream beside the feature-MLP backward (it needs only
-// the scans, and touches none of the MLP backward's buffers).
+// wave then scatters 4 samples at a time.
 constexpr int kCompBwdWaves = 4;
-constexpr int kCompBoth = 0, kCompScan = 1, kCompScatter = 2;
 
-template <int PHASE>
 __global__ void __launch_bounds__(64 * kCompBwdWaves) k_composite_bwd(tfh_vm dvm, BwdIn in,
                                                                      float* __restrict__ dc,
-                                                                     tfh_vm_grad dgrad, int nlc,
-                                                                     float* __restrict__ ds) {
+                                                                     tfh_vm_grad dgrad, int nlc) {
   // nlc consecutive copies of the line gradient: ray r adds into copy r % nlc
   dgrad.line += (int64_t)(blockIdx.x % nlc) * 3 * dvm.grid_dim * dvm.channels;
   extern __shared__ float sm[];
   const int64_t r = blockIdx.x;
   const int lane = lane_id(), wave = threadIdx.x >> 6;
   const int N = in.N, k = in.k;
-  float* s_ds = sm + 4 * N;  // dL/ds_i
-  if constexpr (PHASE != kCompScatter) {
-    float* s_c = sm;              // (N, 3) colour of selected samples, 0 otherwise
-    float* s_sel = s_c + 3 * N;   // 1 for selected samples
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-      s_c[3 * i] = 0.f;
-      s_c[3 * i + 1] = 0.f;
-      s_c[3 * i + 2] = 0.f;
-      s_sel[i] = 0.f;
-    }
-    __syncthreads();
-    const float e_last = in.ray[4 * r], ssum = in.ray[4 * r + 1], u = in.ray[4 * r + 2];
-    const float g0 = in.gout[3 * r], g1 = in.gout[3 * r + 1], g2 = in.gout[3 * r + 2];
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
-      const int i = in.idx ? in.idx[r * k + j] : j;
-      const float* c = in.rgb + (r * k + j) * 3;
-      s_c[3 * i] = c[0];
-      s_c[3 * i + 1] = c[1];
-      s_c[3 * i + 2] = c[2];
-      s_sel[i] = 1.0f;
+  float* s_c = sm;              // (N, 3) colour of selected samples, 0 otherwise
+  float* s_sel = s_c + 3 * N;   // 1 for selected samples
+  float* s_ds = s_sel + N;      // dL/ds_i
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    s_c[3 * i] = 0.f;
+    s_c[3 * i + 1] = 0.f;
+    s_c[3 * i + 2] = 0.f;
+    s_sel[i] = 0.f;
+  }
+  __syncthreads();
+  const float e_last = in.ray[4 * r], ssum = in.ray[4 * r + 1], u = in.ray[4 * r + 2];
+  const float g0 = in.gout[3 * r], g1 = in.gout[3 * r + 1], g2 = in.gout[3 * r + 2];
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    const int i = in.idx ? in.idx[r * k + j] : j;
+    const float* c = in.rgb + (r * k + j) * 3;
+    s_c[3 * i] = c[0];
+    s_c[3 * i + 1] = c[1];
+    s_c[3 * i + 2] = c[2];
+    s_sel[i] = 1.0f;
+    const float wv = in.w[r * N + i];
+    const float f = u * wv;
+    dc[(r * k + j) * 3] = f * g0;
+    dc[(r * k + j) * 3 + 1] = f * g1;
+    dc[(r * k + j) * 3 + 2] = f * g2;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    // A = sum_K w c
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int i = lane; i < N; i += 64) {
       const float wv = in.w[r * N + i];
-      const float f = u * wv;
-      dc[(r * k + j) * 3] = f * g0;
-      dc[(r * k + j) * 3 + 1] = f * g1;
-      dc[(r * k + j) * 3 + 2] = f * g2;
+      a0 += wv * s_c[3 * i];
+      a1 += wv * s_c[3 * i + 1];
+      a2 += wv * s_c[3 * i + 2];
     }
-    __syncthreads();
-    if (wave == 0) {
-      // A = sum_K w c
-      float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-      for (int i = lane; i < N; i += 64) {
-        const float wv = in.w[r * N + i];
-        a0 += wv * s_c[3 * i];
-        a1 += wv * s_c[3 * i + 1];
-        a2 += wv * s_c[3 * i + 2];
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    const float inv_s = 1.0f / (ssum + 1e-8f);
+    const float gsum = g0 + g1 + g2;
+    const float gA = g0 * a0 + g1 * a1 + g2 * a2;
+    // reverse chunks: exclusive suffix sums of (w c) and (w sel)
+    float c0s = 0.f, c1s = 0.f, c2s = 0.f, css = 0.f;  // carries from later chunks
+    const int nchunks = (N + 63) / 64;
+    for (int cb = nchunks - 1; cb >= 0; --cb) {
+      const int i = cb * 64 + lane;
+      const bool act = i < N;
+      const float wv = act ? in.w[r * N + i] : 0.f;
+      const float sel = act ? s_sel[i] : 0.f;
+      const float cx = act ? s_c[3 * i] : 0.f, cy = act ? s_c[3 * i + 1] : 0.f,
+                  cz = act ? s_c[3 * i + 2] : 0.f;
+      float v0 = wv * cx, v1 = wv * cy, v2 = wv * cz, vs = wv * sel;
+      // inclusive suffix within the chunk
+  #pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const float t0 = __shfl_down(v0, o, 64), t1 = __shfl_down(v1, o, 64),
+                    t2 = __shfl_down(v2, o, 64), ts = __shfl_down(vs, o, 64);
+        if (lane + o < 64) {
+          v0 += t0;
+          v1 += t1;
+          v2 += t2;
+          vs += ts;
+        }
       }
-      a0 = wave_sum(a0);
-      a1 = wave_sum(a1);
-      a2 = wave_sum(a2);
-      const float inv_s = 1.0f / (ssum + 1e-8f);
-      const float gsum = g0 + g1 + g2;
-      const float gA = g0 * a0 + g1 * a1 + g2 * a2;
-      // reverse chunks: exclusive suffix sums of (w c) and (w sel)
-      float c0s = 0.f, c1s = 0.f, c2s = 0.f, css = 0.f;  // carries from later chunks
-      const int nchunks = (N + 63) / 64;
-      for (int cb = nchunks - 1; cb >= 0; --cb) {
-        const int i = cb * 64 + lane;
-        const bool act = i < N;
-        const float wv = act ? in.w[r * N + i] : 0.f;
-        const float sel = act ? s_sel[i] : 0.f;
-        const float cx = act ? s_c[3 * i] : 0.f, cy = act ? s_c[3 * i + 1] : 0.f,
-                    cz = act ? s_c[3 * i + 2] : 0.f;
-        float v0 = wv * cx, v1 = wv * cy, v2 = wv * cz, vs = wv * sel;
-        // inclusive suffix within the chunk
-    #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const float t0 = __shfl_down(v0, o, 64), t1 = __shfl_down(v1, o, 64),
-                      t2 = __shfl_down(v2, o, 64), ts = __shfl_down(vs, o, 64);
-          if (lane + o < 64) {
-            v0 += t0;
-            v1 += t1;
-            v2 += t2;
-            vs += ts;
-          }
-        }
-        v0 += c0s;
-        v1 += c1s;
-        v2 += c2s;
-        vs += css;
-        float x0 = __shfl_down(v0, 1, 64), x1 = __shfl_down(v1, 1, 64), x2 = __shfl_down(v2, 1, 64),
-              xs = __shfl_down(vs, 1, 64);
-        if (lane == 63) {
-          x0 = c0s;
-          x1 = c1s;
-          x2 = c2s;
-          xs = css;
-        }
-        if (act) {
-          const float ei = in.e[r * N + i];
-          const float dA0 = sel * ei * cx - x0, dA1 = sel * ei * cy - x1, dA2 = sel * ei * cz - x2;
-          const float dS = sel * ei - xs;
-          const float dU = e_last * inv_s - u * dS * inv_s;
-          const float dl_da = dU * gA + u * (g0 * dA0 + g1 * dA1 + g2 * dA2) - e_last * gsum;
-          const float sig = in.sigma[r * N + i];
-          const float dsi = dl_da * in.delta[r * N + i] * (-expm1f(-sig));
-          if constexpr (PHASE == kCompScan)
-            ds[r * N + i] = dsi;
-          else
-            s_ds[i] = dsi;
-        }
-        c0s = __shfl(v0, 0, 64);
-        c1s = __shfl(v1, 0, 64);
-        c2s = __shfl(v2, 0, 64);
-        css = __shfl(vs, 0, 64);
+      v0 += c0s;
+      v1 += c1s;
+      v2 += c2s;
+      vs += css;
+      float x0 = __shfl_down(v0, 1, 64), x1 = __shfl_down(v1, 1, 64), x2 = __shfl_down(v2, 1, 64),
+            xs = __shfl_down(vs, 1, 64);
+      if (lane == 63) {
+        x0 = c0s;
+        x1 = c1s;
+        x2 = c2s;
+        xs = css;
       }
+      if (act) {
+        const float ei = in.e[r * N + i];
+        const float dA0 = sel * ei * cx - x0, dA1 = sel * ei * cy - x1, dA2 = sel * ei * cz - x2;
+        const float dS = sel * ei - xs;
+        const float dU = e_last * inv_s - u * dS * inv_s;
+        const float dl_da = dU * gA + u * (g0 * dA0 + g1 * dA1 + g2 * dA2) - e_last * gsum;
+        const float sig = in.sigma[r * N + i];
+        s_ds[i] = dl_da * in.delta[r * N + i] * (-expm1f(-sig));
+      }
+      c0s = __shfl(v0, 0, 64);
+      c1s = __shfl(v1, 0, 64);
+      c2s = __shfl(v2, 0, 64);
+      css = __shfl(vs, 0, 64);
     }
   }
-  if constexpr (PHASE == kCompScan) return;
-  if constexpr (PHASE == kCompBoth) __syncthreads();
+  __syncthreads();
   // density scatter: every channel of every mode receives dL/ds_i
   const int grp = lane >> 4, l16 = lane & 15;
   const int G = (int)dvm.grid_dim, C = (int)dvm.channels;
@@ -399,7 +384,7 @@ __global__ void __launch_bounds__(64 * kCompBwdWaves) k_composite_bwd(tfh_vm dvm
       xh = p[0];
       yh = p[1];
       zh = p[2];
-      g = PHASE == kCompScatter ? ds[r * N + i] : s_ds[i];
+      g = s_ds[i];
     }
     ModeTaps mt[3];
     mode_taps(xh, yh, zh, G, mt);
@@ -506,8 +491,6 @@ struct RenderWs {
   float* dlcopy = nullptr;
   float* alcopy = nullptr;
   int dnlc = 1, anlc = 1;
-  // dL/ds per sample (R, N) when the density scatter runs on a second stream
-  float* dsig = nullptr;
 };
 
 static int32_t check_cfg(const tfh_render_config* cfg, const char* who) {
@@ -608,53 +591,6 @@ static void take_line_copies(Carve& cv, const tfh_vm* density, const tfh_vm* app
   }
 }
 
-// The training steps run the density scatter of the compositing backward on a
-// second stream while the feature-MLP backward runs on the caller's: one is
-// bound by memory-side atomics, the other by matrix-core issue. The size
-// queries reserve dL/ds per sample for the hand-over.
-static size_t split_bytes(const tfh_render_config* cfg, int64_t R) {
-  return align_up((size_t)R * cfg->density_samples_per_ray * sizeof(float), 256);
-}
-
-static void take_split(Carve& cv, const tfh_render_config* cfg, int64_t R, RenderWs& ws) {
-  ws.dsig = nullptr;
-  const size_t bytes = split_bytes(cfg, R);
-  if (!cv.ok || cv.off + bytes > cv.cap) return;
-  ws.dsig = cv.take<float>(R * cfg->density_samples_per_ray);
-}
-
-// One side stream (default priority) and a fork / join event pair per device.
-// The mutex is held while a backward enqueues its two branches, so that calls
-// from several host threads do not interleave their event records.
-struct SideStream {
-  std::mutex mu;
-  hipStream_t st = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-  bool ready = false, failed = false;
-};
-
-static SideStream* side_stream() {
-  static SideStream sides[64];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
-    (void)hipGetLastError();
-    return nullptr;
-  }
-  SideStream& s = sides[dev];
-  std::lock_guard<std::mutex> lk(s.mu);
-  if (!s.ready && !s.failed) {
-    if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) == hipSuccess &&
-        hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) == hipSuccess &&
-        hipEventCreateWithFlags(&s.join, hipEventDisableTiming) == hipSuccess) {
-      s.ready = true;
-    } else {
-      (void)hipGetLastError();
-      s.failed = true;
-    }
-  }
-  return s.ready ? &s : nullptr;
-}
-
 static RayIn make_rayin(const tfh_render_config* cfg, const float* aabb, const float* o,
                         const float* d, int64_t R, const float* jitter, const float* base_ts,
                         const float* base_steps, const float* gumbel) {
@@ -712,20 +648,6 @@ static int32_t render_forward(const tfh_vm* density, const tfh_vm* appearance, c
   return part ? sum_copies(ws.mse_part, kLossSlots, 1, mse, st) : TFH_OK;
 }
 
-static int32_t appearance_scatter(const tfh_vm* appearance, int64_t M, int k, int N,
-                                  const RenderWs& ws, const tfh_vm_grad* agrad, hipStream_t st) {
-  const int64_t aln = 3 * appearance->grid_dim * appearance->channels;
-  RaySamples src{ws.d.pts, ws.d.idx, k, N};
-  ProfScope ps("appearance_scatter", st);
-  if (!ws.alcopy)
-    return launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, *agrad, st);
-  TFH_TRY(zero_copies(ws.alcopy, ws.anlc * aln, st));
-  const tfh_vm_grad ag{agrad->plane, ws.alcopy};
-  TFH_TRY((launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, ag, st, 1,
-                                               ws.anlc)));
-  return sum_copies(ws.alcopy, ws.anlc, aln, agrad->line, st);
-}
-
 static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
                                const tfh_mlp* mlp, const tfh_render_config* cfg, int64_t R,
                                const float* dirs, const uint32_t* cams, const float* gout,
@@ -735,46 +657,31 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
   const int64_t M = R * k;
   BwdIn bi{ws.d.pts, ws.d.sigma, ws.d.delta, ws.d.w, ws.d.e, ws.d.ray, ws.rgb, gout, ws.d.idx, N, k};
   const int64_t dln = 3 * density->grid_dim * density->channels;
-  tfh_vm_grad dg = *dgrad;
-  if (ws.dlcopy) dg.line = ws.dlcopy;
-  const int dnlc = ws.dlcopy ? ws.dnlc : 1;
-  // per-stage timing keeps every kernel on the caller's stream
-  SideStream* side = (ws.dsig && !profiling()) ? side_stream() : nullptr;
-  if (!side) {
-    {
-      ProfScope ps("composite_bwd_density_scatter", st);
-      if (ws.dlcopy) TFH_TRY(zero_copies(ws.dlcopy, ws.dnlc * dln, st));
-      hipLaunchKernelGGL(k_composite_bwd<kCompBoth>, dim3((unsigned)R), dim3(64 * kCompBwdWaves),
-                         5 * N * sizeof(float), st, *density, bi, ws.dc, dg, dnlc, nullptr);
-      TFH_TRY(check_launch("render bwd: compositing"));
-      if (ws.dlcopy) TFH_TRY(sum_copies(ws.dlcopy, ws.dnlc, dln, dgrad->line, st));
+  const int64_t aln = 3 * appearance->grid_dim * appearance->channels;
+  {
+    ProfScope ps("composite_bwd_density_scatter", st);
+    tfh_vm_grad dg = *dgrad;
+    if (ws.dlcopy) {
+      TFH_TRY(zero_copies(ws.dlcopy, ws.dnlc * dln, st));
+      dg.line = ws.dlcopy;
     }
-    TFH_TRY(mlp_bwd_rows(mlp, ws.feats, dirs, cams, M, k, ws.dc, ws.dfeat, mgrad,
-                         cfg->mlp_precision, ws.mlp_ws, ws.mlp_ws_bytes, st, ws.act,
-                         ws.act ? ws.rgb : nullptr));
-    return appearance_scatter(appearance, M, k, N, ws, agrad, st);
+    hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)R), dim3(64 * kCompBwdWaves),
+                       5 * N * sizeof(float), st, *density, bi, ws.dc, dg,
+                       ws.dlcopy ? ws.dnlc : 1);
+    TFH_TRY(check_launch("render bwd: compositing"));
+    if (ws.dlcopy) TFH_TRY(sum_copies(ws.dlcopy, ws.dnlc, dln, dgrad->line, st));
   }
-  std::lock_guard<std::mutex> lk(side->mu);
-  hipLaunchKernelGGL(k_composite_bwd<kCompScan>, dim3((unsigned)R), dim3(64 * kCompBwdWaves),
-                     5 * N * sizeof(float), st, *density, bi, ws.dc, dg, dnlc, ws.dsig);
-  TFH_TRY(check_launch("render bwd: compositing scans"));
-  TFH_HIP_TRY(hipEventRecord(side->fork, st));
-  TFH_HIP_TRY(hipStreamWaitEvent(side->st, side->fork, 0));
-  if (ws.dlcopy) TFH_TRY(zero_copies(ws.dlcopy, ws.dnlc * dln, side->st));
-  hipLaunchKernelGGL(k_composite_bwd<kCompScatter>, dim3((unsigned)R), dim3(64 * kCompBwdWaves),
-                     0, side->st, *density, bi, ws.dc, dg, dnlc, ws.dsig);
-  TFH_TRY(check_launch("render bwd: density scatter"));
-  if (ws.dlcopy) TFH_TRY(sum_copies(ws.dlcopy, ws.dnlc, dln, dgrad->line, side->st));
-  TFH_HIP_TRY(hipEventRecord(side->join, side->st));
-  const int32_t rc = [&]() -> int32_t {
-    TFH_TRY(mlp_bwd_rows(mlp, ws.feats, dirs, cams, M, k, ws.dc, ws.dfeat, mgrad,
-                         cfg->mlp_precision, ws.mlp_ws, ws.mlp_ws_bytes, st, ws.act,
-                         ws.act ? ws.rgb : nullptr));
-    return appearance_scatter(appearance, M, k, N, ws, agrad, st);
-  }();
-  // join even after an error, so that later work on st stays ordered
-  TFH_HIP_TRY(hipStreamWaitEvent(st, side->join, 0));
-  return rc;
+  TFH_TRY(mlp_bwd_rows(mlp, ws.feats, dirs, cams, M, k, ws.dc, ws.dfeat, mgrad, cfg->mlp_precision,
+                       ws.mlp_ws, ws.mlp_ws_bytes, st, ws.act, ws.act ? ws.rgb : nullptr));
+  RaySamples src{ws.d.pts, ws.d.idx, k, N};
+  ProfScope ps("appearance_scatter", st);
+  if (!ws.alcopy)
+    return launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, *agrad, st);
+  TFH_TRY(zero_copies(ws.alcopy, ws.anlc * aln, st));
+  const tfh_vm_grad ag{agrad->plane, ws.alcopy};
+  TFH_TRY((launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, ag, st, 1,
+                                               ws.anlc)));
+  return sum_copies(ws.alcopy, ws.anlc, aln, agrad->line, st);
 }
 
 static int32_t check_render_args(const tfh_vm* density, const tfh_vm* appearance,
@@ -898,7 +805,7 @@ size_t tfh_train_step_workspace_size(const tfh_render_config* cfg, const tfh_mlp
   Carve cv(nullptr, 0);
   RenderWs ws;
   return carve_render(cv, cfg, mlp, 3 * appearance_channels, num_rays, ws) +
-         act_bytes(cfg, mlp, num_rays) + line_copy_bytes() + split_bytes(cfg, num_rays);
+         act_bytes(cfg, mlp, num_rays) + line_copy_bytes();
 }
 
 int32_t tfh_render_fwd(const tfh_vm* density, const tfh_vm* appearance, const tfh_mlp* mlp,
@@ -983,7 +890,6 @@ int32_t tfh_train_step(const tfh_vm* density, const tfh_vm* appearance, const tf
                 workspace_bytes, cv.off);
   take_act(cv, cfg, mlp, num_rays, ws);
   take_line_copies(cv, density, appearance, ws);
-  take_split(cv, cfg, num_rays, ws);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mse_out, (int64_t)1, 0.0f);
   TFH_TRY(check_launch("train step: clear mse"));
   RayIn in = make_rayin(cfg, aabb, origins, directions, num_rays, jitter, base_ts, base_steps,
@@ -1012,7 +918,7 @@ size_t tfh_train_step_proposal_workspace_size(const tfh_render_config* cfg, cons
   carve_render(cv, cfg, mlp, 3 * appearance_channels, num_rays, ws);
   cv.take<float>(num_rays * (cfg->density_samples_per_ray + 1));
   return cv.off + prop_workspace_bytes(pc, field, num_rays) + act_bytes(cfg, mlp, num_rays) +
-         line_copy_bytes() + split_bytes(cfg, num_rays);
+         line_copy_bytes();
 }
 
 int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
@@ -1070,7 +976,6 @@ int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
                 cv.off);
   take_act(cv, cfg, mlp, R, ws);
   take_line_copies(cv, density, appearance, ws);
-  take_split(cv, cfg, R, ws);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mse_out, (int64_t)1, 0.0f);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, prop_loss_out, (int64_t)1, 0.0f);
   TFH_TRY(check_launch("train step: clear losses"));

@@ -136,13 +136,9 @@ def test_render_backward(n, k, ncam, contraction):
         assert e.max() <= 2e-4 * np.abs(grads[name]).max() + 1e-7, name
 
 
-@pytest.mark.parametrize("staged", [False, True])
-def test_train_step_matches_oracle(staged):
-    """One training step against the oracle's. By default the density scatter of
-    the compositing backward runs on a second stream beside the feature-MLP
-    backward; with per-stage timing on, every kernel stays on one stream."""
-    log(f"case: train step staged={staged}")
-    from tensorf_amd import training, cameras, _lib
+def test_train_step_matches_oracle():
+    log("case: train step")
+    from tensorf_amd import training, cameras
     from tensorf_amd import tensor_vm
     shape = orc.MlpShape()
     g, cd, ca, r, n, k = 20, 16, 48, 256, 64, 16
@@ -170,14 +166,7 @@ def test_train_step_matches_oracle(staged):
     from tensorf_amd import render
     seed = render._seed(st.prng_key, 0)
     jit = render.uniform(render._seed(seed, 1), n).cpu().numpy().astype(np.float64)
-    _lib.profile_enable(staged)
-    try:
-        st, logs = st.training_step(batch)
-        torch.cuda.synchronize()
-        if staged:
-            assert "composite_bwd_density_scatter" in _lib.profile_read()
-    finally:
-        _lib.profile_enable(False)
+    st, logs = st.training_step(batch)
     mse = float(logs["mse"].item())
     ocfg = orc.RenderConfig(0.05, 200.0, orc.RGB, n, k)
     ostate = orc.init_opt_state(p)
