@@ -257,11 +257,30 @@ __global__ void __launch_bounds__(kPNW * 64, 2) k_prop_mlp_bwd(PropW w, const fl
   if (threadIdx.x == 0) part[pp.b3] = db3;
 }
 
-__global__ void k_prop_reduce(const float* __restrict__ partials, int64_t nblk, PropPack pp,
-                              tfh_prop_mlp_grad g) {
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < pp.b3 + 1; e += gridDim.x * blockDim.x) {
+// Sum of the workgroup partials: a workgroup owns 64 consecutive entries and
+// its kPRedWaves waves split the partials (wave w adds rows w, w + kPRedWaves,
+// ...), so every load is one coalesced 256-byte row segment; the waves' sums
+// meet in LDS and are added in a fixed order.
+constexpr int kPRedWaves = 16;
+
+__global__ void __launch_bounds__(64 * kPRedWaves) k_prop_reduce(const float* __restrict__ partials,
+                                                                int64_t nblk, PropPack pp,
+                                                                tfh_prop_mlp_grad g) {
+  __shared__ float red[kPRedWaves][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int n = pp.b3 + 1;
+  const int e = blockIdx.x * 64 + lane;
+  float part = 0.0f;
+  if (e < n) {
+#pragma unroll 8
+    for (int64_t b = w; b < nblk; b += kPRedWaves) part += partials[b * pp.total + e];
+  }
+  red[w][lane] = part;
+  __syncthreads();
+  if (w == 0 && e < n) {
     float s = 0.0f;
-    for (int64_t b = 0; b < nblk; ++b) s += partials[b * pp.total + e];
+#pragma unroll
+    for (int i = 0; i < kPRedWaves; ++i) s += red[i][lane];
     float* dst;
     int o;
     if (e >= pp.b3) { dst = g.b3; o = e - pp.b3; }
@@ -334,8 +353,8 @@ int32_t prop_mlp_bwd(const tfh_prop_mlp* p, const float* feats, const float* dz,
                        feats, dz, M, idx, count, dfeat, partials);
   }
   TFH_TRY(check_launch("proposal mlp bwd"));
-  hipLaunchKernelGGL(k_prop_reduce, dim3((unsigned)ceil_div(pp.b3 + 1, 256)), dim3(256), 0, st,
-                     partials, nblk, pp, *g);
+  hipLaunchKernelGGL(k_prop_reduce, dim3((unsigned)ceil_div(pp.b3 + 1, 64)),
+                     dim3(64 * kPRedWaves), 0, st, partials, nblk, pp, *g);
   return check_launch("proposal mlp reduce");
 }
 
