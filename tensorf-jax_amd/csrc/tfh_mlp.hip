@@ -5,13 +5,30 @@
 namespace tfh {
 
 // grads[e] += sum_b partials[b][e], mapped onto the separate gradient arrays.
-__global__ void k_mlp_reduce(const float* __restrict__ partials, int64_t nblk, GradPack gp,
-                             tfh_mlp_grad g) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < gp.total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    if (e >= gp.b3 + 3) continue;
+// Sum of the workgroup partials: a workgroup owns 64 consecutive entries and
+// its kRedWaves waves split the partials (wave w adds rows w, w + kRedWaves,
+// ...), so every load is one coalesced 256-byte row segment; the waves' sums
+// meet in LDS and are added in a fixed order.
+constexpr int kRedWaves = 16;
+
+__global__ void __launch_bounds__(64 * kRedWaves) k_mlp_reduce(const float* __restrict__ partials,
+                                                              int64_t nblk, GradPack gp,
+                                                              tfh_mlp_grad g) {
+  __shared__ float red[kRedWaves][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t n = gp.b3 + 3;
+  const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+  float part = 0.0f;
+  if (e < n) {
+#pragma unroll 8
+    for (int64_t b = w; b < nblk; b += kRedWaves) part += partials[b * gp.total + e];
+  }
+  red[w][lane] = part;
+  __syncthreads();
+  if (w == 0 && e < n) {
     float s = 0.0f;
-    for (int64_t b = 0; b < nblk; ++b) s += partials[b * gp.total + e];
+#pragma unroll
+    for (int i = 0; i < kRedWaves; ++i) s += red[i][lane];
     float* dst;
     int64_t o;
     if (e >= gp.b3) { dst = g.b3; o = e - gp.b3; }
@@ -218,8 +235,8 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   }
   TFH_TRY(check_launch("mlp bwd"));
   ProfScope ps("mlp_reduce", st);
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)ceil_div(gp.total, 256)), dim3(256), 0, st,
-                     partials, nblk, gp, *g);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)ceil_div(gp.b3 + 3, 64)),
+                     dim3(64 * kRedWaves), 0, st, partials, nblk, gp, *g);
   return check_launch("mlp bwd: reduce");
 }
 
