@@ -78,13 +78,33 @@ __global__ void __launch_bounds__(256) k_zero_copies(float* __restrict__ p, int6
   }
 }
 
-__global__ void __launch_bounds__(256) k_sum_copies(const float* __restrict__ copies, int ncopy,
-                                                    int64_t len, float* __restrict__ dst) {
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < len;
-       e += (int64_t)gridDim.x * 256) {
-    float sum = 0.0f;
-    for (int k = 0; k < ncopy; ++k) sum += copies[k * len + e];
-    dst[e] += sum;
+// A workgroup adds up 64 consecutive entries at a time; its kSumWaves waves
+// split the copies (wave w adds copies w, w + kSumWaves, ...), so every load
+// is one coalesced 256-byte segment, and the waves' sums meet in LDS in a
+// fixed order. The chunk loop's trip count is the same for every thread.
+constexpr int kSumWaves = 16;
+
+__global__ void __launch_bounds__(64 * kSumWaves) k_sum_copies(const float* __restrict__ copies,
+                                                              int ncopy, int64_t len,
+                                                              float* __restrict__ dst) {
+  __shared__ float red[kSumWaves][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int64_t e0 = (int64_t)blockIdx.x * 64; e0 < len; e0 += (int64_t)gridDim.x * 64) {
+    const int64_t e = e0 + lane;
+    float part = 0.0f;
+    if (e < len) {
+#pragma unroll 4
+      for (int k = w; k < ncopy; k += kSumWaves) part += copies[(int64_t)k * len + e];
+    }
+    red[w][lane] = part;
+    __syncthreads();
+    if (w == 0 && e < len) {
+      float sum = 0.0f;
+#pragma unroll
+      for (int i = 0; i < kSumWaves; ++i) sum += red[i][lane];
+      dst[e] += sum;
+    }
+    __syncthreads();
   }
 }
 
@@ -98,9 +118,9 @@ int32_t zero_copies(float* p, int64_t n, hipStream_t st) {
 
 int32_t sum_copies(const float* copies, int ncopy, int64_t len, float* dst, hipStream_t st) {
   if (len <= 0) return TFH_OK;
-  const int64_t blocks = std::min<int64_t>(ceil_div(len, 256), 4096);
-  hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)blocks), dim3(256), 0, st, copies, ncopy, len,
-                     dst);
+  const int64_t blocks = std::min<int64_t>(ceil_div(len, 64), 8192);
+  hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)blocks), dim3(64 * kSumWaves), 0, st, copies,
+                     ncopy, len, dst);
   return check_launch("sum gradient copies");
 }
 
