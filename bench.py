@@ -70,6 +70,33 @@ def stage_model(stage, R, N, k, cd, ca, precision, n_params, prop=None):
     return None
 
 
+def compulsory_bytes(stage, R, N, k, cd, ca, G, n_params):
+    """HBM bytes a launch cannot avoid: every input read once, every output
+    written once, the factors (or their gradients, read and written) once --
+    no tap re-reads. Achieved / 8 TB/s on this model is at most 1 for any
+    kernel, so it grades the memory-bound stages (the logical 72*C bytes per
+    sample of the stage model counts cache hits and can exceed the peak)."""
+    M = R * k
+    sel = 4 if k < N else 0  # top-k indices
+    dfac = 4 * (3 * G * G * cd + 3 * G * cd)
+    afac = 4 * (3 * G * G * ca + 3 * G * ca)
+    if stage == "density_pass":
+        # rays in; points, distances, sigma, step, weight, transmittance out;
+        # per-ray terms and the selection
+        return R * 24 + dfac + R * N * 32 + R * 16 + M * sel
+    if stage == "appearance_gather":
+        return M * (12 + sel) + afac + M * 4 * 3 * ca
+    if stage == "composite_bwd_density_scatter":
+        # per sample: point, sigma, step, weight, transmittance; per shaded
+        # sample: colour in, dL/dc out; per ray: terms and dL/dC; gradients RMW
+        return R * N * 28 + M * (24 + sel) + R * 28 + 2 * dfac
+    if stage == "appearance_scatter":
+        return M * (12 + sel + 4 * 3 * ca) + afac + 2 * afac
+    if stage == "adam":
+        return n_params * 32
+    return None
+
+
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -90,29 +117,91 @@ def make_batches(R, count, seed, device):
     return out
 
 
-def cpu_baseline(R_sample, N, k, G, cd, ca):
-    """The oracle (NumPy FP32 restatement of the reference) timed for one
-    training step on R_sample rays of the same workload, single-threaded."""
+_CPU = {}
+
+
+def _cpu_worker_init(G, N, k, cd, ca, R_total):
+    """One CPU-baseline worker: its own copy of the parameters and rays."""
     import numpy as np
     from threadpoolctl import threadpool_limits
     from oracle import tensorf_oracle as orc
+    _CPU["limits"] = threadpool_limits(1)
     shape = orc.MlpShape()
     p = orc.make_params(G, cd, ca, shape, seed=0)
-    p = {key: v.astype(np.float32) for key, v in p.items()}
-    o, d = orc.make_rays(R_sample, seed=1, radius=4.0, target_half=0.8)
-    colors = np.random.default_rng(2).uniform(size=(R_sample, 3))
-    cfg = orc.RenderConfig(0.05, 200.0, orc.RGB, N, k)
-    st = orc.init_opt_state(p)
-    jit = np.random.default_rng(3).uniform(size=N)
-    with threadpool_limits(1):
+    _CPU.update(orc=orc, shape=shape, R=R_total,
+                p={key: v.astype(np.float32) for key, v in p.items()},
+                rays=orc.make_rays(R_total, seed=1, radius=4.0, target_half=0.8),
+                colors=np.random.default_rng(2).uniform(size=(R_total, 3)).astype(np.float32),
+                cfg=orc.RenderConfig(0.05, 200.0, orc.RGB, N, k),
+                jit=np.random.default_rng(3).uniform(size=N).astype(np.float32))
+
+
+def _cpu_worker(span):
+    """Forward and analytic backward of the rays [a, b) (render.py:105-279 and
+    its VJP) in float32: the squared error sum and the parameter gradients of
+    the batch-mean loss."""
+    import numpy as np
+    c = _CPU
+    orc = c["orc"]
+    a, b = span
+    o, d = c["rays"]
+    box = np.array([[-1.5] * 3, [1.5] * 3], np.float32)
+    out, cache = orc.render_rays(c["p"], box, o[a:b], d[a:b], np.zeros(b - a), c["cfg"],
+                                 c["shape"], c["jit"], dtype=np.float32)
+    diff = out - c["colors"][a:b]
+    gout = (2.0 * diff / (3 * c["R"])).astype(np.float32)
+    grads = orc.render_rays_backward(c["p"], cache, gout, c["cfg"], c["shape"], np.float32)
+    return float((diff.astype(np.float64) ** 2).sum()), grads
+
+
+def cpu_baseline(R_sample, N, k, G, cd, ca, cores):
+    """The oracle (NumPy float32 restatement of the reference) timed for one
+    training step of R_sample rays of the workload on `cores` host cores: the
+    rays are split into one contiguous part per core, each worker process runs
+    the forward and analytic backward of its part single-threaded, and the
+    parent sums the gradients and runs Adam on every parameter. Must run before
+    the process touches the GPU (the workers are forked)."""
+    import multiprocessing as mp
+    import platform
+
+    import numpy as np
+    from oracle import tensorf_oracle as orc
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    bounds = np.linspace(0, R_sample, cores + 1).astype(int)
+    spans = [(int(bounds[i]), int(bounds[i + 1])) for i in range(cores)]
+    ctx = mp.get_context("fork")
+    with ctx.Pool(cores, initializer=_cpu_worker_init,
+                  initargs=(G, N, k, cd, ca, R_sample)) as pool:
+        pool.map(_cpu_worker, [(0, 1)] * cores)  # every worker initialised and warm
         t0 = time.perf_counter()
-        orc.train_step(p, st, 0, np.array([[-1.5] * 3, [1.5] * 3]), o, d, np.zeros(R_sample),
-                       colors, cfg, shape, jit, n_iters=30000, dtype=np.float32)
-        dt = time.perf_counter() - t0
-    return {"value": R_sample / dt, "unit": "rays/s", "cores": 1, "kind": "port",
-            "sample": f"one oracle training step (fwd + analytic bwd + Adam over all params) on "
-                      f"{R_sample} of the {N}-sample rays at G={G}, NumPy float32, 1 thread; "
-                      f"{dt:.1f} s"}
+        parts = pool.map(_cpu_worker, spans)
+        sq = sum(pp[0] for pp in parts)
+        grads = {key: sum(pp[1][key] for pp in parts) for key in parts[0][1]}
+        t_grad = time.perf_counter()
+        _cpu_worker_init(G, N, k, cd, ca, R_sample)  # the parent's parameters for Adam
+        t_init = time.perf_counter()
+        opt = orc.OptimizerConfig()
+        params = _CPU["p"]
+        for key, g in grads.items():
+            lr = opt.lr_init_mlp if key in orc.MLP_KEYS else opt.lr_init_tensor
+            orc.adam_update(params[key], g, np.zeros_like(g), np.zeros_like(g), 1, lr, 1.0,
+                            dtype=np.float32)
+        dt = (t_grad - t0) + (time.perf_counter() - t_init)
+    mse = sq / (3 * R_sample)
+    return {"value": R_sample / dt, "unit": "rays/s", "cores": cores, "kind": "port",
+            "cpu_model": model,
+            "sample": f"one oracle training step (fwd + analytic bwd + Adam over all "
+                      f"{sum(v.size for v in grads.values())} params) on {R_sample} of the "
+                      f"{N}-sample rays at G={G}, NumPy float32, the rays split over {cores} "
+                      f"worker processes of one thread each; {dt:.1f} s, mse {mse:.5f}"}
 
 
 def bench_proposal(args, cd, ca, G, n_out=64):
@@ -213,7 +302,8 @@ def fisheye_batches(R, count, n_frames, seed, device):
     return out
 
 
-def bench_leg(args, label, R, N, k, G, cd, ca, precision, contraction=False, n_frames=0):
+def bench_leg(args, label, R, N, k, G, cd, ca, precision, contraction=False, n_frames=0,
+              spacing="reference"):
     """Uniform-sampled training and rendering on one further workload; returns
     the numbers merged into the JSON line."""
     import torch
@@ -224,9 +314,10 @@ def bench_leg(args, label, R, N, k, G, cd, ca, precision, contraction=False, n_f
         initial_aabb_max=(box,) * 3, appearance_feat_dim=ca, density_feat_dim=cd,
         feature_n_freqs=2, viewdir_n_freqs=2, grid_dim_init=G, grid_dim_final=G, upsamp_iters=(),
         scene_contraction=contraction, camera_embeddings=n_frames > 0,
-        density_samples_per_ray=N, appearance_samples_per_ray=k, mlp_precision=precision)
+        density_samples_per_ray=N, appearance_samples_per_ray=k, mlp_precision=precision,
+        contracted_spacing=spacing)
     progress(f"{label}: init state G={G} R={R} N={N} k={k} {precision} "
-             f"contraction={contraction} cameras={n_frames}")
+             f"contraction={contraction} ({spacing} spacing) cameras={n_frames}")
     state = training.TrainState.initialize(cfg, G, prng_key=0, num_cameras=max(n_frames, 1))
     if contraction:
         mbs = fisheye_batches(R, 4, n_frames, seed=9, device="cuda")
@@ -286,7 +377,11 @@ def main():
     ap.add_argument("--appearance-samples", type=int, default=512)
     ap.add_argument("--grid", type=int, default=300)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
-    ap.add_argument("--cpu-sample-rays", type=int, default=2048)
+    ap.add_argument("--cpu-sample-rays", type=int, default=0,
+                    help="rays of the CPU baseline's training step (default: 1024 per core)")
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="host cores for the CPU baseline (default: the affinity set, at "
+                         "most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stages", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -302,6 +397,16 @@ def main():
               "(data parallelism over rays is a design note, not code). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    # the CPU baseline forks its workers, so it runs before anything touches the GPU
+    cpu = None
+    if not args.no_cpu_baseline:
+        cores = args.cpu_cores or min(len(os.sched_getaffinity(0)), 16)
+        r_cpu = args.cpu_sample_rays or 1024 * cores
+        progress(f"cpu baseline: {r_cpu} rays on {cores} cores")
+        cpu = cpu_baseline(r_cpu, args.samples, args.appearance_samples, args.grid, 16, 48,
+                           cores)
+        progress(f"cpu baseline: {cpu['value']:.1f} rays/s")
 
     import torch
     from tensorf_amd import _lib, cameras, render, training
@@ -400,7 +505,22 @@ def main():
             roofline = {"kernel": name, "bound": bound, "achieved": round(achieved, 3),
                         "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
                         "traffic": traffic, "avg_ms": round(avg, 4),
-                        "algorithmic_per_launch": amount}
+                        "algorithmic_per_launch": amount,
+                        "traffic_source": (f"profiles/traffic_{name}.json: rocprofv3 --pmc "
+                                           f"FETCH_SIZE and WRITE_SIZE of this workload, "
+                                           f"scaled by a copy of known size (not measured "
+                                           f"in this run)") if traffic is not None else None}
+        hbm = {}
+        for name, (ms, cnt) in rec.items():
+            b = compulsory_bytes(name, R, N, k, cd, ca, G, n_params)
+            if b is None:
+                continue
+            avg = ms / max(cnt, 1)
+            gbs = b / (avg * 1e-3) / 1e9
+            hbm[name] = {"compulsory_bytes": int(b), "avg_ms": round(avg, 4),
+                         "achieved_GBs": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4)}
+        if hbm:
+            roofline["hbm_compulsory"] = hbm
 
     prop = prop128 = topk = unb = None
     del state, mbs, batches, rays, lp, ws
@@ -411,11 +531,12 @@ def main():
                             f"per-ray top-64 by weight shaded by the BF16 MLP, VM {G}^3")
         torch.cuda.empty_cache()
         unb = bench_leg(args, "unbounded", 16384, 512, 512, G, cd, ca, "bf16",
-                        contraction=True, n_frames=200)
+                        contraction=True, n_frames=200, spacing="uniform_contracted")
         unb["workload"] = (f"unbounded training step: L-inf scene contraction, R=16384 "
                            f"equidistant-fisheye rays (960x540 frames, 200 ring cameras, "
-                           f"FiLM camera embedding) x 512 samples uniform in contracted "
-                           f"distance, all shaded by the BF16 MLP, VM {G}^3")
+                           f"FiLM camera embedding) x 512 samples spaced uniformly in "
+                           f"contracted distance g(t) = t (t <= 1), 2 - 1/t (t > 1), all "
+                           f"shaded by the BF16 MLP, VM {G}^3")
         torch.cuda.empty_cache()
     if not args.no_proposal:
         prop = bench_proposal(args, cd, ca, G, 64)
@@ -423,11 +544,6 @@ def main():
         for p in (prop, prop128):
             p["train_speedup_vs_this_config"] = round(p["train_rays_per_s"] / value, 2)
             p["render_speedup_vs_this_config"] = round(p["render_rays_per_s"] / render_value, 2)
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        progress(f"cpu baseline on {args.cpu_sample_rays} rays")
-        cpu = cpu_baseline(args.cpu_sample_rays, N, k, G, cd, ca)
 
     line = {
         "metric": METRIC,

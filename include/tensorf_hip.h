@@ -47,8 +47,20 @@ enum tfh_status {
   TFH_ERR_WORKSPACE = -5
 };
 
-/* render.RenderMode (render.py:18-23) */
-enum tfh_render_mode { TFH_MODE_RGB = 0, TFH_MODE_DIST_MEDIAN = 1, TFH_MODE_DIST_MEAN = 2 };
+/* render.RenderMode (render.py:18-23).
+ * TFH_MODE_DIST_MEDIAN: the distance of the first sample whose 1 - E_i
+ *   exceeds 0.5 (render.py:248-267), 0 when that is sample 0 (the xor of the
+ *   padded mask never selects column 0) and +inf when no sample reaches it.
+ * TFH_MODE_DIST_MEDIAN_AS_WRITTEN: render.py:248-267 evaluated as written
+ *   under IEEE arithmetic: the 0/1 mask multiplies a distance row that ends in
+ *   +inf, and 0 * inf = NaN, so a ray whose 1 - E_{N-1} exceeds 0.5 gets NaN
+ *   and any other ray +inf. For parity checks against the reference. */
+enum tfh_render_mode {
+  TFH_MODE_RGB = 0,
+  TFH_MODE_DIST_MEDIAN = 1,
+  TFH_MODE_DIST_MEAN = 2,
+  TFH_MODE_DIST_MEDIAN_AS_WRITTEN = 3
+};
 
 /* Arithmetic of the appearance MLP. FP32: FP32 MFMA. BF16: BF16 operands on
  * MFMA with FP32 accumulation. */
@@ -413,6 +425,59 @@ int32_t tfh_train_step_proposal(const tfh_vm* density, const tfh_vm* appearance,
                                 float lr_mlp, float decay, int64_t adam_count, float* mse_out,
                                 float* prop_loss_out, void* workspace, size_t workspace_bytes,
                                 tfh_stream_t stream);
+
+/* ---- XLA custom calls ------------------------------------------------------
+ * render_rays's forward and VJP and the fused training step in XLA's original
+ * GPU custom-call form (api_version 0), so that the reference's JAX loop,
+ * which differentiates render.render_rays (render.py:105-279) inside
+ * jax.value_and_grad (training.py:153-156), can call the kernels through
+ * jax.ffi.register_ffi_target(name, capsule, platform="ROCM", api_version=0)
+ * without XLA headers:
+ *   void f(hipStream_t stream, void** buffers, const char* opaque,
+ *          size_t opaque_len);
+ * buffers holds the device pointers of the operands in operand order, then
+ * of the results in result order. opaque is an ASCII descriptor (XLA's
+ * backend_config) of space-separated key=value fields:
+ *   tfhx=1 near far mode N k contraction precision sampling shift   (tfh_render_config)
+ *   R dG dC aG aC squash units nf nv ncam                          (rays, factors, MLP)
+ *   jitter schedule gumbel (0/1: operand present) ws (workspace bytes)
+ *   offsets=o0,...,o11 total tcount                   (tfh_xla_train_step only)
+ * floats printed with enough digits to round-trip float32. Factor tensors
+ * are channel-last as everywhere in this header. An absent optional operand
+ * (embedding, jitter, schedule, gumbel) still takes its slot; any buffer does.
+ * The form has no status channel: on a bad descriptor or a failed call the
+ * entry point fills its float results with NaN, prints the message to stderr
+ * and records it as the calling thread's tfh_last_error().
+ *
+ * tfh_xla_render_fwd (tfh_render_fwd):
+ *   operands 0 density plane, 1 density line, 2 appearance plane,
+ *     3 appearance line, 4 w0, 5 w1, 6 b1, 7 w2, 8 b2, 9 w3, 10 b3, 11 embed,
+ *     12 aabb (2,3), 13 origins (R,3), 14 directions (R,3),
+ *     15 camera_indices (R) u32, 16 jitter, 17 base_ts, 18 base_steps, 19 gumbel
+ *   results 20 out (R,3) or (R), 21 workspace u8[ws] (ws =
+ *     tfh_render_workspace_size; the residuals the VJP reads)
+ * tfh_xla_render_bwd (tfh_render_bwd, RGB mode):
+ *   operands 0-15 as above, 16 grad_out (R,3), 17 the forward's workspace
+ *   results 18 density plane grad, 19 density line grad, 20 appearance plane
+ *     grad, 21 appearance line grad, 22-28 w0 b... grads in operand order
+ *     4-10, 29 embed grad, 30 workspace u8[ws] (alias it to operand 17 with
+ *     input_output_aliases; otherwise operand 17 is copied into it first)
+ *   The gradients are written, not accumulated.
+ * tfh_xla_train_step (tfh_train_step):
+ *   operands 0 params, 1 exp_avg, 2 exp_avg_sq (flat f32[total], views at
+ *     offsets: density plane, line, appearance plane, line, w0, w1, b1, w2,
+ *     b2, w3, b3, embed; the first tcount floats use the factor rate),
+ *     3 aabb, 4 origins, 5 directions, 6 camera_indices, 7 colors (R,3),
+ *     8 jitter, 9 base_ts, 10 base_steps, 11 gumbel,
+ *     12 hyper f32[4] = {-lr_tensor * decay, -lr_mlp * decay, 1 - 0.9^count,
+ *     1 - 0.99^count} (the Adam step's scalars, computed by the caller)
+ *   results 13 params, 14 exp_avg, 15 exp_avg_sq (alias them to operands
+ *     0-2; otherwise the operands are copied first), 16 mse f32[1],
+ *     17 gradient scratch f32[total], 18 workspace u8[ws] (ws =
+ *     tfh_train_step_workspace_size) */
+void tfh_xla_render_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+void tfh_xla_render_bwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+void tfh_xla_train_step(void* stream, void** buffers, const char* opaque, size_t opaque_len);
 
 /* ---- synthetic scenes ----------------------------------------------------- */
 /* Ground truth of a procedural scene (replaces the image datasets of data.py):

@@ -31,6 +31,7 @@ import math
 from typing import Dict, Optional, Tuple
 
 import numpy as np
+import scipy.sparse
 
 # Mode permutations of TensorVM.interpolate: ijk, kij, jki (tensor_vm.py:50-52).
 MODE_PERMS = ((0, 1, 2), (2, 0, 1), (1, 2, 0))
@@ -119,14 +120,37 @@ def contraction_schedule(samples_per_ray, near, far):
     return ts, steps
 
 
+def uniform_contracted_schedule(samples_per_ray, near, far):
+    """Sample distances spaced uniformly in contracted distance
+    g(t) = t (t <= 1), 2 - 1/t (t > 1) -- the contraction's radial map for a
+    ray leaving the scene centre -- between near and far: t_i = g^-1(s_i),
+    s_i evenly spaced from g(near) to g(far) in float64, cast to float32; step
+    sizes as in the reference schedule (differences, the last one repeated,
+    render.py:154-155). No counterpart in the reference (its schedule is
+    contraction_schedule); the alternative spacing of the unbounded
+    configuration. Returns (ts (N,) float32, step_sizes (N,) float32)."""
+    n = int(samples_per_ray)
+    g = lambda t: t if t <= 1.0 else 2.0 - 1.0 / t
+    s = np.linspace(g(near), g(far), n)
+    ts = np.where(s <= 1.0, s, 1.0 / (2.0 - np.minimum(s, 2.0 - 1e-300))).astype(np.float32)
+    steps = (np.roll(ts, -1) - ts).astype(np.float32)
+    steps[-1] = steps[-2] if n > 1 else np.float32(far - near)
+    return ts, steps
+
+
+SCHEDULES = {"reference": contraction_schedule, "uniform_contracted": uniform_contracted_schedule}
+
+
 def sample_contracted(origins, directions, samples_per_ray, near, far, jitter,
-                      dtype=np.float64):
+                      dtype=np.float64, spacing="reference"):
     """Contracted-scene sampler + L-infinity contraction, render.py:122-173.
 
     ``jitter`` (R, N) is per ray and per sample (render.py:158-160).
+    ``spacing`` picks the un-jittered schedule: the reference's
+    (contraction_schedule) or uniform_contracted_schedule.
     Returns points (3, R, N) in contracted space, ts (R, N), step_sizes (R, N).
     """
-    base, steps1 = contraction_schedule(samples_per_ray, near, far)
+    base, steps1 = SCHEDULES[spacing](samples_per_ray, near, far)
     o = _c(origins, dtype)
     d = _c(directions, dtype)
     r = o.shape[0]
@@ -213,14 +237,24 @@ def vm_interpolate(vector, matrix, ijk, dtype=np.float64):
     g = mat.shape[-1]
     outs = []
     for m in range(3):
-        (v0, v1, wv0, wv1), (a0, a1, wa0, wa1), (b0, b1, wb0, wb1) = _mode_taps(p, g, m, dtype)
-        line = wv0 * vec[m][:, v0] + wv1 * vec[m][:, v1]
-        # JAX's map_coordinates sums the 2-D taps in product order
-        # (a0,b0),(a0,b1),(a1,b0),(a1,b1), each as (wa*wb)*value.
-        plane = ((((wa0 * wb0) * mat[m][:, a0, b0] + (wa0 * wb1) * mat[m][:, a0, b1])
-                  + (wa1 * wb0) * mat[m][:, a1, b0]) + (wa1 * wb1) * mat[m][:, a1, b1])
+        line, plane = _line_plane(vec[m], mat[m], _mode_taps(p, g, m, dtype))
         outs.append(line * plane)                                    # tensor_vm.py:167
     return np.concatenate(outs, axis=0).astype(dtype)               # tensor_vm.py:86-88
+
+
+def _line_plane(vec, mat, taps):
+    """Line and plane values (C, M) of one mode at its taps. JAX's
+    map_coordinates sums the 2-D taps in product order (a0,b0),(a0,b1),
+    (a1,b0),(a1,b1), each as (wa*wb)*value. The texels are read as rows of
+    the transposed tables (the same values, gathered row-wise)."""
+    (v0, v1, wv0, wv1), (a0, a1, wa0, wa1), (b0, b1, wb0, wb1) = taps
+    c, g = vec.shape
+    vt = np.ascontiguousarray(vec.T)                                 # (G, C)
+    mt = np.ascontiguousarray(mat.reshape(c, g * g).T)               # (G*G, C)
+    line = (wv0[:, None] * vt[v0] + wv1[:, None] * vt[v1]).T
+    plane = ((((wa0 * wb0)[:, None] * mt[a0 * g + b0] + (wa0 * wb1)[:, None] * mt[a0 * g + b1])
+              + (wa1 * wb0)[:, None] * mt[a1 * g + b0]) + (wa1 * wb1)[:, None] * mt[a1 * g + b1]).T
+    return line, plane
 
 
 def vm_interpolate_vjp(vector, matrix, ijk, grad_features, dtype=np.float64):
@@ -235,18 +269,28 @@ def vm_interpolate_vjp(vector, matrix, ijk, grad_features, dtype=np.float64):
     gvec = np.zeros_like(vec)
     gmat = np.zeros_like(mat)
     for m in range(3):
-        (v0, v1, wv0, wv1), (a0, a1, wa0, wa1), (b0, b1, wb0, wb1) = _mode_taps(p, g, m, dtype)
-        line = wv0 * vec[m][:, v0] + wv1 * vec[m][:, v1]
-        plane = ((((wa0 * wb0) * mat[m][:, a0, b0] + (wa0 * wb1) * mat[m][:, a0, b1])
-                  + (wa1 * wb0) * mat[m][:, a1, b0]) + (wa1 * wb1) * mat[m][:, a1, b1])
+        taps = _mode_taps(p, g, m, dtype)
+        (v0, v1, wv0, wv1), (a0, a1, wa0, wa1), (b0, b1, wb0, wb1) = taps
+        line, plane = _line_plane(vec[m], mat[m], taps)
         gm = gf[m * c:(m + 1) * c]
         gline = gm * plane
         gplane = gm * line
-        np.add.at(gvec[m], (slice(None), v0), wv0 * gline)
-        np.add.at(gvec[m], (slice(None), v1), wv1 * gline)
+        # The scatter-add is the transposed tap matrix times the per-sample
+        # gradients: T[texel, sample] = tap weight, duplicate taps (clamped at
+        # the grid edge) summed.
+        n = p.shape[1]
+        cols = np.arange(n)
+        tv = scipy.sparse.csr_matrix((np.concatenate([wv0, wv1]), (
+            np.concatenate([v0, v1]), np.concatenate([cols, cols]))), shape=(g, n))
+        gvec[m] += (tv @ gline.T).T
+        rows_p, w_p = [], []
         for ia, wa in ((a0, wa0), (a1, wa1)):
             for ib, wb in ((b0, wb0), (b1, wb1)):
-                np.add.at(gmat[m], (slice(None), ia, ib), (wa * wb) * gplane)
+                rows_p.append(ia * g + ib)
+                w_p.append(wa * wb)
+        tp = scipy.sparse.csr_matrix((np.concatenate(w_p), (
+            np.concatenate(rows_p), np.tile(cols, 4))), shape=(g * g, n))
+        gmat[m] += (tp @ gplane.T).T.reshape(c, g, g)
     return gvec, gmat
 
 
@@ -493,13 +537,16 @@ class RenderConfig:
     scene_contraction: bool = False
     sampling: int = 0  # 0: the reference's samplers, 1: per-ray interval edges
     density_shift: float = 10.0  # render.py:207
+    median_as_written: bool = False  # DIST_MEDIAN: evaluate render.py:250-266 literally
+    contracted_spacing: str = "reference"  # or "uniform_contracted"
 
 
 SAMPLING_SCHEDULE, SAMPLING_EDGES = 0, 1
 
 
 def render_rays(params, aabb, origins, directions, camera_indices, cfg: RenderConfig,
-                shape: MlpShape, jitter, gumbel=None, dtype=np.float64, edges=None):
+                shape: MlpShape, jitter, gumbel=None, dtype=np.float64, edges=None,
+                mlp_operand_round=None):
     """render.render_rays, render.py:105-279 with _rgb_from_points (render.py:437-549).
 
     params: dict with 'density_vector' (3,Cd,G), 'density_matrix' (3,Cd,G,G),
@@ -507,6 +554,9 @@ def render_rays(params, aabb, origins, directions, camera_indices, cfg: RenderCo
     init_mlp_params. jitter: (N,) bbox mode or (R, N) contraction mode.
     With cfg.sampling == SAMPLING_EDGES the samples come from ``edges`` (R, N+1)
     (sample_from_edges) and jitter is (R, N) or None.
+    mlp_operand_round (e.g. bf16_round) rounds the operands of the MLP's matrix
+    products like a reduced-precision MFMA (feature_mlp_forward); the backward
+    applies the same rounding.
     Returns (output, cache): output (R,3) for RGB, (R,) for the depth modes."""
     o = _c(origins, dtype)
     d = _c(directions, dtype)
@@ -515,7 +565,8 @@ def render_rays(params, aabb, origins, directions, camera_indices, cfg: RenderCo
     if cfg.sampling == SAMPLING_EDGES:
         pts, ts, steps = sample_from_edges(o, d, edges, jitter, cfg.scene_contraction, dtype)
     elif cfg.scene_contraction:
-        pts, ts, steps = sample_contracted(o, d, n, cfg.near, cfg.far, jitter, dtype)
+        pts, ts, steps = sample_contracted(o, d, n, cfg.near, cfg.far, jitter, dtype,
+                                           cfg.contracted_spacing)
     else:
         pts, ts, steps = sample_points_along_ray_within_bbox(o, d, aabb, n, jitter, dtype)
     xhat = normalize_points(pts, aabb, dtype)                        # render.py:193-195
@@ -526,11 +577,24 @@ def render_rays(params, aabb, origins, directions, camera_indices, cfg: RenderCo
     p_exits, w = compute_segment_probabilities(sigma, steps, dtype)  # render.py:211
     cache = dict(xhat=xhat, ts=ts, steps=steps, s=s, sigma=sigma, p_exits=p_exits, w=w, d=d,
                  cams=np.asarray(camera_indices, np.int64))
+    if cfg.mode == DIST_MEDIAN and cfg.median_as_written:
+        # render.py:250-266 literally: pad the distances with +inf and the
+        # "not alive" probabilities with 1, xor neighbouring mask entries, and
+        # sum mask * distance under IEEE rules (0 * inf = NaN).
+        dist = np.concatenate([ts, np.full((r, 1), np.inf, dtype)], axis=1)
+        alive_not = np.concatenate([1.0 - p_exits, np.ones((r, 1), dtype)], axis=1)
+        mask = alive_not > 0.5
+        sel = np.zeros_like(mask)
+        sel[:, 1:] = np.logical_xor(mask[:, :-1], mask[:, 1:])
+        with np.errstate(invalid="ignore"):
+            out = (sel.astype(dtype) * dist).sum(axis=1)
+        return out.astype(dtype), cache
     if cfg.mode == DIST_MEDIAN:
         # render.py:248-267: the first sample whose 1-E exceeds 0.5; 0 when it is
         # sample 0 (xor quirk), +inf when never reached. The reference multiplies
-        # a 0/1 mask by a distance row that ends in +inf; this restates the
-        # intended selection (a select, not 0*inf=NaN) -- parity unpinned.
+        # a 0/1 mask by a distance row that ends in +inf; this is the selection
+        # the finite columns of that sum make (median_as_written evaluates the
+        # sum itself).
         above = (1.0 - p_exits) > 0.5
         first = np.where(above.any(axis=1), above.argmax(axis=1), n)
         out = np.where(first == n, np.inf, np.where(first == 0, 0.0,
@@ -547,7 +611,8 @@ def render_rays(params, aabb, origins, directions, camera_indices, cfg: RenderCo
                            dtype).T                                  # render.py:476-486
     vd = np.repeat(d, k, axis=0)                                     # render.py:487-490
     cams = np.repeat(np.asarray(camera_indices, np.int64), k)        # render.py:492-496
-    rgb, mlp_cache = feature_mlp_forward(params, afeat, vd, cams, shape, dtype)  # render.py:499-509
+    rgb, mlp_cache = feature_mlp_forward(params, afeat, vd, cams, shape, dtype,
+                                         mlp_operand_round)          # render.py:499-509
     rgb = rgb.reshape(r, k, 3)
     wsel = w[rows, idx]                                              # render.py:529-531
     e_last = p_exits[:, -1]

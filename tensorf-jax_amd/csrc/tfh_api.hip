@@ -135,7 +135,13 @@ __global__ void __launch_bounds__(256) k_adam(float* __restrict__ p, float* __re
                                               float* __restrict__ m, float* __restrict__ v,
                                               int64_t n, int64_t split, float lr_lo, float lr_hi,
                                               float b1, float b2, float bc1, float bc2, float eps,
-                                              int zero_grad) {
+                                              int zero_grad, const float* __restrict__ hyper) {
+  if (hyper) {  // the step's scalars from device memory: -lr lo, -lr hi, 1-b1^t, 1-b2^t
+    lr_lo = hyper[0];
+    lr_hi = hyper[1];
+    bc1 = hyper[2];
+    bc2 = hyper[3];
+  }
   int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
   int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
   for (; i < n; i += stride) {
@@ -179,7 +185,7 @@ __global__ void __launch_bounds__(256) k_adam(float* __restrict__ p, float* __re
 
 int32_t adam_launch(float* p, float* g, float* m, float* v, int64_t n, int64_t split,
                     float lr_lo, float lr_hi, float decay, float b1, float b2, float eps,
-                    int64_t count, int zero_grad, hipStream_t st) {
+                    int64_t count, int zero_grad, hipStream_t st, const float* hyper) {
   // the learning rate enters negated (gradient descent) and scaled by the decay
   float s_lo = -lr_lo * decay, s_hi = -lr_hi * decay;
   float bc1 = 1.0f - powf(b1, (float)count);
@@ -189,7 +195,7 @@ int32_t adam_launch(float* p, float* g, float* m, float* v, int64_t n, int64_t s
   if (blocks < 1) blocks = 1;
   ProfScope ps("adam", st);
   hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(256), 0, st, p, g, m, v, n, split, s_lo,
-                     s_hi, b1, b2, bc1, bc2, eps, zero_grad);
+                     s_hi, b1, b2, bc1, bc2, eps, zero_grad, hyper);
   return check_launch("adam");
 }
 
@@ -298,7 +304,7 @@ int32_t tfh_adam_step(float* param, float* grad, float* exp_avg, float* exp_avg_
   TFH_REQUIRE(count >= 1, "tfh_adam_step: count must be >= 1 (got %lld)", (long long)count);
   TFH_TRY(require_device());
   return adam_launch(param, grad, exp_avg, exp_avg_sq, n, n, lr, lr, decay, b1, b2, eps, count,
-                     zero_grad, as_stream(stream));
+                     zero_grad, as_stream(stream), nullptr);
 }
 
 }  // extern "C"

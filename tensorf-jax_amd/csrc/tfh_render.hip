@@ -32,9 +32,11 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
                      const tfh_mlp_grad* g, int32_t precision, void* ws, size_t ws_bytes,
                      hipStream_t st, const float* act = nullptr, const float* rgb = nullptr);
 bool mlp_keeps_activations(const tfh_mlp* p, int32_t precision);
+// hyper (optional, device): -lr_lo * decay, -lr_hi * decay, 1 - b1^count,
+// 1 - b2^count, read by the kernel in place of the host values.
 int32_t adam_launch(float* p, float* g, float* m, float* v, int64_t n, int64_t split, float lr_lo,
                     float lr_hi, float decay, float b1, float b2, float eps, int64_t count,
-                    int zero_grad, hipStream_t st);
+                    int zero_grad, hipStream_t st, const float* hyper = nullptr);
 
 // ---- per-ray selection (one wave) -----------------------------------------------
 // Top-k of the keys (LDS, order-preserving uint32) -> ascending indices
@@ -165,6 +167,13 @@ __global__ void __launch_bounds__(64) k_density_pass(tfh_vm dvm, RayIn in, Densi
     }
     acc = wave_sum(acc);
     if (lane == 0) o.out[r] = acc;
+    return;
+  }
+  if (in.mode == TFH_MODE_DIST_MEDIAN_AS_WRITTEN) {
+    // render.py:250-266 as written: sum(mask * [ts, +inf]) with mask[N] =
+    // (1 - E_{N-1} <= 0.5); the last column is 0 * inf = NaN or 1 * inf, and
+    // the finite columns cannot undo either.
+    if (lane == 0) o.out[r] = (1.0f - e_last) > 0.5f ? NAN : INFINITY;
     return;
   }
   if (in.mode == TFH_MODE_DIST_MEDIAN) {  // render.py:248-267
@@ -505,7 +514,7 @@ static int32_t check_cfg(const tfh_render_config* cfg, const char* who) {
   TFH_REQUIRE(cfg->sampling == TFH_SAMPLING_SCHEDULE || cfg->sampling == TFH_SAMPLING_EDGES,
               "%s: unknown sampling %d", who, cfg->sampling);
   TFH_REQUIRE(cfg->mode == TFH_MODE_RGB || cfg->mode == TFH_MODE_DIST_MEDIAN ||
-                  cfg->mode == TFH_MODE_DIST_MEAN,
+                  cfg->mode == TFH_MODE_DIST_MEAN || cfg->mode == TFH_MODE_DIST_MEDIAN_AS_WRITTEN,
               "%s: unknown mode %d", who, cfg->mode);
   TFH_REQUIRE(cfg->mlp_precision == TFH_PREC_FP32 || cfg->mlp_precision == TFH_PREC_BF16,
               "%s: unknown precision %d", who, cfg->mlp_precision);
@@ -709,6 +718,54 @@ static int32_t check_render_args(const tfh_vm* density, const tfh_vm* appearance
   return TFH_OK;
 }
 
+// tfh_train_step; with hyper != null the Adam step reads its scalars from
+// device memory (see adam_launch) instead of lr_tensor, lr_mlp, decay and
+// adam_count.
+int32_t train_step_run(const tfh_vm* density, const tfh_vm* appearance, const tfh_mlp* mlp,
+                       const tfh_render_config* cfg, const float* aabb, const float* origins,
+                       const float* directions, const uint32_t* camera_indices,
+                       const float* colors, int64_t num_rays, const float* jitter,
+                       const float* base_ts, const float* base_steps, const float* gumbel,
+                       const tfh_vm_grad* density_grad, const tfh_vm_grad* appearance_grad,
+                       const tfh_mlp_grad* mlp_grad, const tfh_train_buffers* buffers,
+                       float lr_tensor, float lr_mlp, float decay, int64_t adam_count,
+                       const float* hyper, float* mse_out, void* workspace,
+                       size_t workspace_bytes, hipStream_t st) {
+  TFH_TRY(check_render_args(density, appearance, mlp, cfg, aabb, origins, directions, num_rays,
+                            jitter, base_ts, base_steps, "tfh_train_step"));
+  TFH_REQUIRE(cfg->mode == TFH_MODE_RGB, "tfh_train_step: the training loss is on RGB");
+  TFH_REQUIRE(num_rays >= 1, "tfh_train_step: empty batch");
+  TFH_REQUIRE(colors && mse_out && buffers && density_grad && appearance_grad && mlp_grad,
+              "tfh_train_step: null argument");
+  TFH_REQUIRE(buffers->params && buffers->grads && buffers->exp_avg && buffers->exp_avg_sq &&
+                  buffers->total > 0 && buffers->tensor_count >= 0 &&
+                  buffers->tensor_count <= buffers->total,
+              "tfh_train_step: bad parameter buffers");
+  TFH_REQUIRE(hyper != nullptr || adam_count >= 1, "tfh_train_step: adam_count must be >= 1");
+  TFH_TRY(require_device());
+  Carve cv(workspace, workspace_bytes);
+  RenderWs ws;
+  carve_render(cv, cfg, mlp, 3 * appearance->channels, num_rays, ws);
+  if (!cv.ok)
+    return fail(TFH_ERR_WORKSPACE, "tfh_train_step: workspace of %zu bytes, need %zu",
+                workspace_bytes, cv.off);
+  take_act(cv, cfg, mlp, num_rays, ws);
+  take_line_copies(cv, density, appearance, ws);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mse_out, (int64_t)1, 0.0f);
+  TFH_TRY(check_launch("train step: clear mse"));
+  RayIn in = make_rayin(cfg, aabb, origins, directions, num_rays, jitter, base_ts, base_steps,
+                        gumbel);
+  // the rendered colours are not needed by the caller: keep them in ws.gout's
+  // neighbour (dc is free until the backward)
+  TFH_TRY(render_forward(density, appearance, mlp, cfg, in, camera_indices, ws.dc, colors, mse_out,
+                         ws, st));
+  TFH_TRY(render_backward(density, appearance, mlp, cfg, num_rays, directions, camera_indices,
+                          ws.gout, density_grad, appearance_grad, mlp_grad, ws, st));
+  return adam_launch(buffers->params, buffers->grads, buffers->exp_avg, buffers->exp_avg_sq,
+                     buffers->total, buffers->tensor_count, lr_tensor, lr_mlp, decay, 0.9f, 0.99f,
+                     1e-8f, hyper ? 1 : adam_count, 1, st, hyper);
+}
+
 }  // namespace tfh
 
 using namespace tfh;
@@ -869,40 +926,10 @@ int32_t tfh_train_step(const tfh_vm* density, const tfh_vm* appearance, const tf
                        float lr_tensor, float lr_mlp, float decay, int64_t adam_count,
                        float* mse_out, void* workspace, size_t workspace_bytes,
                        tfh_stream_t stream) {
-  TFH_TRY(check_render_args(density, appearance, mlp, cfg, aabb, origins, directions, num_rays,
-                            jitter, base_ts, base_steps, "tfh_train_step"));
-  TFH_REQUIRE(cfg->mode == TFH_MODE_RGB, "tfh_train_step: the training loss is on RGB");
-  TFH_REQUIRE(num_rays >= 1, "tfh_train_step: empty batch");
-  TFH_REQUIRE(colors && mse_out && buffers && density_grad && appearance_grad && mlp_grad,
-              "tfh_train_step: null argument");
-  TFH_REQUIRE(buffers->params && buffers->grads && buffers->exp_avg && buffers->exp_avg_sq &&
-                  buffers->total > 0 && buffers->tensor_count >= 0 &&
-                  buffers->tensor_count <= buffers->total,
-              "tfh_train_step: bad parameter buffers");
-  TFH_REQUIRE(adam_count >= 1, "tfh_train_step: adam_count must be >= 1");
-  TFH_TRY(require_device());
-  hipStream_t st = as_stream(stream);
-  Carve cv(workspace, workspace_bytes);
-  RenderWs ws;
-  carve_render(cv, cfg, mlp, 3 * appearance->channels, num_rays, ws);
-  if (!cv.ok)
-    return fail(TFH_ERR_WORKSPACE, "tfh_train_step: workspace of %zu bytes, need %zu",
-                workspace_bytes, cv.off);
-  take_act(cv, cfg, mlp, num_rays, ws);
-  take_line_copies(cv, density, appearance, ws);
-  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mse_out, (int64_t)1, 0.0f);
-  TFH_TRY(check_launch("train step: clear mse"));
-  RayIn in = make_rayin(cfg, aabb, origins, directions, num_rays, jitter, base_ts, base_steps,
-                        gumbel);
-  // the rendered colours are not needed by the caller: keep them in ws.gout's
-  // neighbour (dc is free until the backward)
-  TFH_TRY(render_forward(density, appearance, mlp, cfg, in, camera_indices, ws.dc, colors, mse_out,
-                         ws, st));
-  TFH_TRY(render_backward(density, appearance, mlp, cfg, num_rays, directions, camera_indices,
-                          ws.gout, density_grad, appearance_grad, mlp_grad, ws, st));
-  return adam_launch(buffers->params, buffers->grads, buffers->exp_avg, buffers->exp_avg_sq,
-                     buffers->total, buffers->tensor_count, lr_tensor, lr_mlp, decay, 0.9f, 0.99f,
-                     1e-8f, adam_count, 1, st);
+  return train_step_run(density, appearance, mlp, cfg, aabb, origins, directions, camera_indices,
+                        colors, num_rays, jitter, base_ts, base_steps, gumbel, density_grad,
+                        appearance_grad, mlp_grad, buffers, lr_tensor, lr_mlp, decay, adam_count,
+                        nullptr, mse_out, workspace, workspace_bytes, as_stream(stream));
 }
 
 size_t tfh_train_step_proposal_workspace_size(const tfh_render_config* cfg, const tfh_mlp* mlp,

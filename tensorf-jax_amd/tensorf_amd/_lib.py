@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtensorf_hip.so")
 
 TFH_OK = 0
-MODE_RGB, MODE_DIST_MEDIAN, MODE_DIST_MEAN = 0, 1, 2
+MODE_RGB, MODE_DIST_MEDIAN, MODE_DIST_MEAN, MODE_DIST_MEDIAN_AS_WRITTEN = 0, 1, 2, 3
 PREC_FP32, PREC_BF16 = 0, 1
 SAMPLING_SCHEDULE, SAMPLING_EDGES = 0, 1
 
@@ -142,6 +142,9 @@ _SIGS = {
                                         P(MlpGrad), P(VmGrad), P(PropMlpGrad), P(TrainBuffers),
                                         P(TrainBuffers), c_f, c_f, c_f, c_i64, c_p, c_p, c_p,
                                         c_sz, c_p]),
+    "tfh_xla_render_fwd": (None, [c_p, P(c_p), ctypes.c_char_p, c_sz]),
+    "tfh_xla_render_bwd": (None, [c_p, P(c_p), ctypes.c_char_p, c_sz]),
+    "tfh_xla_train_step": (None, [c_p, P(c_p), ctypes.c_char_p, c_sz]),
     "tfh_uniform": (c_i32, [ctypes.c_uint64, ctypes.c_uint64, c_i64, c_p, c_p]),
     "tfh_blob_scene_render": (c_i32, [c_p, c_i32, c_p, c_p, c_i64, ctypes.c_double,
                                       ctypes.c_double, c_i32, c_i32, c_p, c_p, c_p, c_p]),

@@ -44,6 +44,15 @@ class RenderConfig:
     # set, density_samples_per_ray must equal proposal.samples_out and
     # LearnableParams.proposal_field holds its parameters.
     proposal: Optional[proposal.ProposalConfig] = None
+    # DIST_MEDIAN only. False: the first sample whose 1 - E exceeds 0.5 (0 for
+    # sample 0, +inf when never reached), what render.py:248-267 selects.
+    # True: render.py:248-267 evaluated as written, where 0 * inf in the
+    # padded last column makes every ray NaN (opacity above 0.5) or +inf.
+    median_as_written: bool = False
+    # Contracted scenes only: the un-jittered sample distances. "reference":
+    # the reference's schedule (render.py:127-155); "uniform_contracted":
+    # spaced uniformly in contracted distance g(t) = t (t <= 1), 2 - 1/t.
+    contracted_spacing: str = "reference"
 
 
 @dataclasses.dataclass
@@ -114,13 +123,32 @@ def contraction_schedule(samples_per_ray: int, near: float, far: float):
     return ts, steps
 
 
+def uniform_contracted_schedule(samples_per_ray: int, near: float, far: float):
+    """Un-jittered distances spaced uniformly in contracted distance
+    g(t) = t (t <= 1), 2 - 1/t (t > 1) between near and far (evaluated in
+    float64, cast to float32); step sizes are the differences with the last
+    one repeated, as in the reference schedule (render.py:154-155). Host-side
+    table setup (N floats)."""
+    n = int(samples_per_ray)
+    g = lambda t: t if t <= 1.0 else 2.0 - 1.0 / t
+    s = np.linspace(g(near), g(far), n)
+    ts = np.where(s <= 1.0, s, 1.0 / (2.0 - np.minimum(s, 2.0 - 1e-300))).astype(np.float32)
+    steps = (np.roll(ts, -1) - ts).astype(np.float32)
+    steps[-1] = steps[-2] if n > 1 else np.float32(far - near)
+    return ts, steps
+
+
+SCHEDULES = {"reference": contraction_schedule, "uniform_contracted": uniform_contracted_schedule}
+
 _SCHEDULE_CACHE: Dict = {}
 
 
-def _schedule_tensors(n, near, far, device):
-    key = (n, float(near), float(far), str(device))
+def _schedule_tensors(n, near, far, device, spacing="reference"):
+    key = (n, float(near), float(far), str(device), spacing)
     if key not in _SCHEDULE_CACHE:
-        ts, st = contraction_schedule(n, near, far)
+        if spacing not in SCHEDULES:
+            raise ValueError(f"unknown contracted spacing {spacing!r}")
+        ts, st = SCHEDULES[spacing](n, near, far)
         _SCHEDULE_CACHE[key] = (torch.tensor(ts, device=device), torch.tensor(st, device=device))
     return _SCHEDULE_CACHE[key]
 
@@ -128,7 +156,10 @@ def _schedule_tensors(n, near, far, device):
 # ---- C structs ------------------------------------------------------------------------
 def c_config(config: RenderConfig, contraction: bool, precision: int) -> _lib.RenderConfig:
     sampling = _lib.SAMPLING_EDGES if config.proposal is not None else _lib.SAMPLING_SCHEDULE
-    return _lib.RenderConfig(float(config.near), float(config.far), config.mode.value,
+    mode = config.mode.value
+    if config.mode is RenderMode.DIST_MEDIAN and config.median_as_written:
+        mode = _lib.MODE_DIST_MEDIAN_AS_WRITTEN
+    return _lib.RenderConfig(float(config.near), float(config.far), mode,
                              int(config.density_samples_per_ray),
                              int(config.appearance_samples_per_ray), int(bool(contraction)),
                              int(precision), sampling, float(config.density_shift))
@@ -184,7 +215,8 @@ class _RenderEnv:
             self.prop_base_ts = self.prop_base_steps = None
             if self.contraction:
                 self.prop_base_ts, self.prop_base_steps = _schedule_tensors(
-                    pc.samples_round0, config.near, config.far, self.device)
+                    pc.samples_round0, config.near, config.far, self.device,
+                    config.contracted_spacing)
             self.prop_jitter = proposal_noise(prng_key, self.r, pc, self.device)
             if sample_proposal:
                 self.sampler = proposal.Sampler(lp.proposal_field, pc, self.aabb, self.o, self.d,
@@ -193,7 +225,8 @@ class _RenderEnv:
                 self.base_ts = self.sampler.edges
         elif self.contraction:
             self.base_ts, self.base_steps = _schedule_tensors(n, config.near, config.far,
-                                                              self.device)
+                                                              self.device,
+                                                              config.contracted_spacing)
         self.cfg = c_config(config, self.contraction, mlp.precision_code if mlp else 0)
         self.dvm = lp.density_tensor.c_struct()
         self.avm = lp.appearance_tensor.c_struct()

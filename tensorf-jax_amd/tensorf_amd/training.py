@@ -66,6 +66,8 @@ class TensorfConfig:
     # sigma = softplus(features + density_shift); +10 is the reference's
     # (render.py:207)
     density_shift: float = 10.0
+    # contracted scenes: "reference" (render.py:127-155) or "uniform_contracted"
+    contracted_spacing: str = "reference"
 
 
 def lr_decay_coeff(step: int, config: TensorfConfig) -> float:
@@ -224,7 +226,8 @@ class TrainState:
         n, k = sample_counts(self.config, self.flat.shapes[0][1])
         return render.RenderConfig(self.config.render_near, self.config.render_far, mode, n, k,
                                    proposal=self.config.proposal,
-                                   density_shift=self.config.density_shift)
+                                   density_shift=self.config.density_shift,
+                                   contracted_spacing=self.config.contracted_spacing)
 
     # ---- training ------------------------------------------------------------------
     def training_step(self, minibatch: cameras.RenderedRays) -> Tuple["TrainState", Dict]:
@@ -482,6 +485,7 @@ def run_training_loop(config: TensorfConfig, dataset, n_iters: Optional[int] = N
         state, logs = state.training_step(next(batches))
         step = state.step
         if log_fn is not None and (step % log_every == 0 or step == total):
+            logs["grid_dim"] = state.flat.shapes[0][1]  # the grid this step trained on
             log_fn(step, float(logs["mse"].item()), logs)
         if step in config.upsamp_iters:
             state = state.resize_grid(upsampled_grid_dim(config, config.upsamp_iters.index(step)))

@@ -62,3 +62,28 @@ def test_workspace_sizes_are_monotone(lib):
     a = lib.lib.tfh_render_workspace_size(ctypes.byref(cfg), None, 48, 1024)
     b = lib.lib.tfh_render_workspace_size(ctypes.byref(cfg), None, 48, 4096)
     assert 0 < a < b
+
+
+def test_xla_entry_points_reject_bad_descriptors(lib):
+    """The XLA custom calls have no status channel: a bad descriptor is
+    reported through tfh_last_error (and stderr) before any buffer is read."""
+    from tensorf_amd import xla_abi
+    bufs = [None] * 31
+    for name in ("tfh_xla_render_fwd", "tfh_xla_render_bwd", "tfh_xla_train_step"):
+        xla_abi.call(name, 0, bufs, b"tfhx=2 near=1")
+        assert "tfhx=1" in lib.last_error()
+        xla_abi.call(name, 0, bufs, b"tfhx=1 near=0.05 far=200")
+        assert "lacks 'mode'" in lib.last_error()
+        xla_abi.call(name, 0, bufs, b"tfhx=1 junk")
+        assert "not key=value" in lib.last_error()
+    cfg = lib.RenderConfig(0.05, 200.0, 0, 64, 16, 0, 0, 0, 10.0)
+    vm = lib.Vm(None, None, 1, 16)  # grid 1: rejected
+    mlp = lib.Mlp(*([None] * 8), 144, 27, 128, 2, 2, 0)
+    opaque = xla_abi.descriptor(cfg, vm, vm, mlp, 8, 0, jitter=False, schedule=False,
+                                gumbel=False)
+    assert opaque.startswith(b"tfhx=1 near=0.05000000074505806 far=200.0 mode=0 N=64 k=16")
+    xla_abi.call("tfh_xla_render_fwd", 0, bufs, opaque)
+    assert "bad sizes" in lib.last_error()
+    xla_abi.call("tfh_xla_train_step", 0, bufs, opaque.replace(b"dG=1", b"dG=8").replace(
+        b"aG=1", b"aG=8"))
+    assert "needs offsets" in lib.last_error()

@@ -11,6 +11,7 @@ import pytest
 import torch
 
 from oracle import tensorf_oracle as orc
+from tests import closed_forms
 from tests.gpu_common import dev, f32, log, mlp_module, oracle_params, report, to_device_params
 
 pytestmark = pytest.mark.gpu
@@ -68,6 +69,31 @@ def test_render_rgb_forward(n, k, contraction):
     assert bad.mean() < 0.02
 
 
+@pytest.mark.parametrize("k", [24, 128])
+def test_render_rgb_uniform_contracted_spacing(k):
+    """The contracted sampler with distances spaced uniformly in contracted
+    distance (the unbounded configuration's spacing) against the oracle."""
+    log(f"case: render rgb, contraction with uniform contracted spacing, N=128 k={k}")
+    from tensorf_amd import render
+    n = 128
+    shape, p, o, d, cams, box = _setup(contraction=True, radius=3.0)
+    lp, rays, mlp = _gpu_objects(shape, p, o, d, cams, True)
+    cfg = render.RenderConfig(0.05, 200.0, render.RenderMode.RGB, n, k,
+                              contracted_spacing="uniform_contracted")
+    with torch.no_grad():
+        out = render.render_rays(mlp, lp, dev(box), rays, 5, cfg).cpu().numpy()
+    jit = _oracle_jitter(5, len(o), n, True)
+    ocfg = orc.RenderConfig(0.05, 200.0, orc.RGB, n, k, True,
+                            contracted_spacing="uniform_contracted")
+    want, cache = orc.render_rays(p, box, o, d, cams, ocfg, shape, jit)
+    err = report("rgb", out, want)
+    bad = np.any(err > 1e-5 * np.abs(want) + 2e-6, axis=1)
+    for r in np.nonzero(bad)[0]:  # only near-tie selections may differ
+        srt = np.sort(cache["w"][r])[::-1]
+        assert k < n and abs(srt[k - 1] - srt[k]) < 1e-5 * max(srt[k - 1], 1e-30), r
+    assert bad.mean() < 0.02
+
+
 @pytest.mark.parametrize("mode", ["median", "mean"])
 def test_render_distance_modes(mode):
     log(f"case: render distance {mode}")
@@ -97,6 +123,51 @@ def test_render_distance_modes(mode):
     else:
         report("mean", out, want)
         np.testing.assert_allclose(out, want, rtol=1e-5, atol=2e-6)
+
+
+@pytest.mark.parametrize("sigma", [0.05, 0.5, 2.0, 40.0])
+def test_distance_modes_closed_form(sigma):
+    """GPU distances on rays through a box of constant density against the
+    closed forms of tests/closed_forms.py (the median as selected and as
+    render.py:248-267 evaluates it, the mean). sigma 0.05 never reaches the
+    median, 40 reaches it at sample 0. Tolerance: 2e-5 relative (FP32 sampling
+    and log-space scan); rays whose median index sits within FP32 rounding of a
+    sample boundary are left out and counted."""
+    log(f"case: distance modes, constant density {sigma}")
+    from tensorf_amd import render
+    shape = orc.MlpShape()
+    n, r = 128, 256
+    p = oracle_params(16, 16, 48, shape, seed=2)
+    dv, dm = closed_forms.constant_density_factors(16, 16, sigma, 10.0)
+    p["density_vector"], p["density_matrix"] = f32(dv), f32(dm)
+    # the density the device sees: softplus of the float32 feature sum
+    sig = float(np.logaddexp(0.0, float(np.float32(48 * np.float32(dv[0, 0, 0]) + 10.0))))
+    o, d = orc.make_rays(r, seed=21, radius=4.0, target_half=0.5)
+    o, d = f32(o), f32(d)
+    box = f32(np.array([[-1.5] * 3, [1.5] * 3]))
+    lp, rays, mlp = _gpu_objects(shape, p, o, d, np.zeros(r, np.int64), False)
+    jit = _oracle_jitter(3, r, n, False)
+    t0, t1 = closed_forms.box_segments(o, d, box)
+    med, asw, mean, x = closed_forms.distance_closed_forms(t0, (t1 - t0) / n, sig, n, jit)
+    got = {}
+    for name, mode, written in (("median", render.RenderMode.DIST_MEDIAN, False),
+                                ("as_written", render.RenderMode.DIST_MEDIAN, True),
+                                ("mean", render.RenderMode.DIST_MEAN, False)):
+        cfg = render.RenderConfig(0.05, 200.0, mode, n, n, median_as_written=written)
+        got[name] = render.render_rays(mlp, lp, dev(box), rays, 3, cfg).cpu().numpy()
+    ok = np.abs(x - np.round(x)) > 1e-4 * np.maximum(x, 1.0)
+    log(f"  rays off a rounding edge: {ok.sum()}/{r}; finite medians {np.isfinite(med).sum()}")
+    assert ok.mean() > 0.9
+    fin = ok & np.isfinite(med)
+    if fin.any():
+        report("median", got["median"][fin], med[fin])
+    assert np.array_equal(np.isinf(got["median"][ok]), np.isinf(med[ok]))
+    np.testing.assert_allclose(got["median"][fin], med[fin], rtol=2e-5, atol=1e-6)
+    edge = np.abs(1.0 - np.exp(-sig * (t1 - t0)) - 0.5) < 1e-4
+    assert np.array_equal(np.isnan(got["as_written"][~edge]), np.isnan(asw[~edge]))
+    assert np.all(np.isnan(got["as_written"]) | np.isposinf(got["as_written"]))
+    report("mean", got["mean"], mean)
+    np.testing.assert_allclose(got["mean"], mean, rtol=2e-5)
 
 
 @pytest.mark.parametrize("n,k,ncam,contraction", [(64, 16, None, False), (48, 48, None, False),

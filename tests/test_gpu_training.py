@@ -1,7 +1,8 @@
 """GPU tests of the training-loop layer: grid upsampling of the factors and the
 Adam moments (training.py:245-276), checkpoint round trip, the ground-truth
-scene renderer, and what the proposal sampler gains over uniform sampling at
-the same number of field samples."""
+scene renderer, chunked full-frame rendering (render.py:49-102,
+render_360.py:122-160), and what the proposal sampler gains over uniform
+sampling at the same number of field samples."""
 import dataclasses
 
 import numpy as np
@@ -202,3 +203,42 @@ def test_train_step_kept_activations_match_recompute(monkeypatch, precision, fil
         frac_bad = float(np.mean(e > 1e-3 * lr))
         log(f"  fraction of {name} updates off by >1e-3 lr: {frac_bad:.2e}")
         assert frac_bad <= 1e-3, name
+
+
+@pytest.mark.parametrize("mode", ["rgb", "median", "mean"])
+def test_render_rays_batched_800x800_chunks(mode):
+    """render_rays_batched (render.py:49-102) of an 800x800 view in 65,536-ray
+    chunks (9 full chunks and a ragged one of 50,176 rays) equals one
+    render_rays call over all 640,000 rays bit for bit (every ray is rendered
+    independently; the stratified jitter is shared by all rays), keeps the
+    (H, W[, 3]) batch axes, and the full-frame driver (render_360.py:122-160)
+    turns it into the same 8-bit image."""
+    log(f"case: 800x800 chunked render, {mode}")
+    from tensorf_amd import cameras, render, render_360, scenes, training
+    st = training.TrainState.initialize(_config(g=48, n=64), 48, prng_key=11, num_cameras=1)
+    m = {"rgb": render.RenderMode.RGB, "median": render.RenderMode.DIST_MEDIAN,
+         "mean": render.RenderMode.DIST_MEAN}[mode]
+    cfg = render.RenderConfig(0.05, 200.0, m, 64, 16)
+    c2w = scenes.hemisphere_cameras(5, radius=4.0, seed=3)[2]
+    rays = cameras.pinhole_rays(c2w, 800, 800, scenes.focal_800(), 0)
+    out = render.render_rays_batched(st.appearance_mlp, st.learnable_params, st.aabb, rays, 4,
+                                     cfg, batch_size=65536)
+    assert out.shape == ((800, 800, 3) if mode == "rgb" else (800, 800))
+    with torch.no_grad():
+        whole = render.render_rays(st.appearance_mlp, st.learnable_params, st.aabb,
+                                   rays.reshape_flat(), 4, cfg).cpu().numpy()
+    whole = whole.reshape(out.shape)
+    log(f"  finite outputs: {np.isfinite(out).mean():.3f}")
+    assert np.array_equal(out, whole, equal_nan=True)
+    frames = list(render_360.render_frames(
+        st, c2w, 800, 800, 2 * np.arctan(400 / scenes.focal_800()), 1, m,
+        density_samples_per_ray=64, appearance_samples_per_ray=16, batch_size=65536,
+        translation_scale=1.0, prng_key=4))
+    want = render_360.rgb_image(out) if mode == "rgb" else render_360.DistanceImages()(out)
+    assert frames[0].shape == (800, 800, 3) and frames[0].dtype == np.uint8
+    # the driver derives the focal length from the field of view (float64
+    # round trip through tan/atan), so a pixel may land on the other side of
+    # an 8-bit rounding step
+    diff = np.abs(frames[0].astype(np.int32) - want.astype(np.int32))
+    log(f"  driver image: {np.mean(diff > 0):.2e} of the bytes differ, by at most {diff.max()}")
+    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3

@@ -8,7 +8,7 @@ import pytest
 import torch
 
 from oracle import tensorf_oracle as orc
-from tests import torch_ref
+from tests import closed_forms, torch_ref
 
 RNG = np.random.default_rng(7)
 BOX = np.array([[-1.5, -1.5, -1.5], [1.5, 1.5, 1.5]])
@@ -189,23 +189,62 @@ def test_render_backward_central_fd():
             assert abs(fd - flat_g[j]) < 1e-6 + 1e-5 * abs(fd), (key, j, fd, flat_g[j])
 
 
-def test_depth_modes():
-    shape, p, o, d, cams, cfg, jit = _small_problem()
-    box = np.array([[-1.0] * 3, [1.0] * 3])
-    cfg.mode = orc.DIST_MEAN
-    mean, c = orc.render_rays(p, box, o, d, cams, cfg, shape, jit)
-    np.testing.assert_allclose(mean, (c["w"] * c["ts"]).sum(1) + c["p_exits"][:, -1] * c["ts"][:, -1])
-    cfg.mode = orc.DIST_MEDIAN
-    med, c = orc.render_rays(p, box, o, d, cams, cfg, shape, jit)
-    for r in range(len(med)):
-        alive = 1 - c["p_exits"][r]
-        above = np.nonzero(alive > 0.5)[0]
-        if len(above) == 0:
-            assert med[r] == np.inf
-        elif above[0] == 0:
-            assert med[r] == 0.0
-        else:
-            assert med[r] == c["ts"][r, above[0]]
+@pytest.mark.parametrize("n,near,far", [(512, 0.05, 200.0), (64, 0.05, 200.0), (33, 1.5, 9.0)])
+def test_uniform_contracted_schedule(n, near, far):
+    """Distances spaced uniformly in contracted distance: g(t) = t (t <= 1),
+    2 - 1/t beyond, is evenly spaced over the schedule from g(near) to g(far);
+    the product's host table is the same array; step sizes are differences
+    with the last one repeated (render.py:154-155)."""
+    from tensorf_amd import render
+    ts, st = orc.uniform_contracted_schedule(n, near, far)
+    g = np.where(ts <= 1.0, ts, 2.0 - 1.0 / ts.astype(np.float64))
+    want = np.linspace(near if near <= 1 else 2 - 1 / near, 2 - 1 / far, n)
+    np.testing.assert_allclose(g, want, atol=3e-7)
+    assert abs(ts[0] - near) < 1e-6 * near and abs(ts[-1] - far) < 1e-5 * far
+    assert np.all(np.diff(ts) > 0)
+    np.testing.assert_array_equal(st[:-1], (ts[1:] - ts[:-1]).astype(np.float32))
+    assert st[-1] == st[-2]
+    pts, tt, ss = orc.sample_contracted(np.zeros((1, 3)), np.array([[0.0, 0.0, 1.0]]), n, near,
+                                        far, np.zeros((1, n)), spacing="uniform_contracted")
+    np.testing.assert_array_equal(tt[0], ts.astype(np.float64))
+    # a ray leaving the centre: the contracted points are evenly spaced in radius
+    np.testing.assert_allclose(pts[2, 0], want, atol=1e-6)
+    hts, hst = render.uniform_contracted_schedule(n, near, far)
+    np.testing.assert_array_equal(hts, ts)
+    np.testing.assert_array_equal(hst, st)
+
+
+@pytest.mark.parametrize("sigma", [0.05, 0.5, 2.0, 40.0])
+@pytest.mark.parametrize("jittered", [False, True])
+def test_depth_modes_closed_form(sigma, jittered):
+    """Median (selection and as written) and mean distance on rays through a
+    box of constant density against the closed forms of tests/closed_forms.py
+    (sigma 0.05: the median is never reached; 40: it is sample 0)."""
+    shape = orc.MlpShape()
+    n, r = 96, 40
+    p = orc.make_params(8, 16, 48, shape, seed=1)
+    p["density_vector"], p["density_matrix"] = closed_forms.constant_density_factors(
+        16, 8, sigma, 10.0)
+    o, d = orc.make_rays(r, seed=12, radius=4.0, target_half=0.5)
+    u = RNG.uniform(size=n) if jittered else np.zeros(n)
+    t0, t1 = closed_forms.box_segments(o, d, BOX)
+    med, asw, mean, x = closed_forms.distance_closed_forms(t0, (t1 - t0) / n, sigma, n, u)
+    ok = np.abs(x - np.round(x)) > 1e-6  # no median index on a rounding knife edge
+    assert ok.mean() > 0.9
+    got = {}
+    for name, mode, written in (("median", orc.DIST_MEDIAN, False),
+                                ("as_written", orc.DIST_MEDIAN, True),
+                                ("mean", orc.DIST_MEAN, False)):
+        cfg = orc.RenderConfig(0.05, 200.0, mode, n, n, median_as_written=written)
+        got[name], _ = orc.render_rays(p, BOX, o, d, np.zeros(r), cfg, shape, u)
+    np.testing.assert_allclose(got["median"][ok], med[ok], rtol=1e-9)
+    assert np.array_equal(np.isnan(got["as_written"]), np.isnan(asw))
+    assert np.all(np.isinf(got["as_written"][~np.isnan(asw)]))
+    np.testing.assert_allclose(got["mean"], mean, rtol=1e-9)
+    if sigma == 0.05:
+        assert np.all(np.isinf(med)) and np.all(np.isinf(got["as_written"]))
+    if sigma == 40.0:
+        assert np.all(med == 0.0) and np.all(np.isnan(got["as_written"]))
 
 
 def test_adam_vs_torch_and_decay():
