@@ -252,6 +252,27 @@ int32_t tfh_render_bwd(const tfh_vm* density, const tfh_vm* appearance, const tf
                        const tfh_vm_grad* appearance_grad, const tfh_mlp_grad* mlp_grad,
                        void* workspace, size_t workspace_bytes, tfh_stream_t stream);
 
+/* tfh_render_fwd without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. The
+ * forward's residuals live in a device array of TFH_RENDER_NOALLOC_SCRATCH_BYTES
+ * inside the library, one per device, held from the first call on that device
+ * until the process ends; the call allocates nothing. A call whose residuals
+ * do not fit returns TFH_ERR_WORKSPACE before any device work: per ray they
+ * take about 32 N + 592 k bytes in RGB mode with 48 appearance channels
+ * (N=512, k=64: ~2,400 rays per call). Results equal tfh_render_fwd's. Calls
+ * on one device share the array: calls that overlap on one device (several
+ * streams or host threads) race. The backward (tfh_render_bwd) needs the
+ * workspace of tfh_render_fwd and cannot follow this call. */
+#define TFH_RENDER_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int32_t tfh_render_fwd_noalloc(const tfh_vm* density, const tfh_vm* appearance,
+                               const tfh_mlp* mlp, const tfh_render_config* cfg,
+                               const float* aabb, const float* origins, const float* directions,
+                               const uint32_t* camera_indices, int64_t num_rays,
+                               const float* jitter, const float* base_ts,
+                               const float* base_steps, const float* gumbel, float* out,
+                               tfh_stream_t stream);
+
 /* ---- optimizer ---------------------------------------------------------- */
 /* One Adam step on a flat group (training.py:184-205, 213-243):
  * optax.scale_by_adam(b1, b2, eps, eps_root=0) -> scale(-lr) -> x decay ->

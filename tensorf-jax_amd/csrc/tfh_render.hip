@@ -718,6 +718,73 @@ static int32_t check_render_args(const tfh_vm* density, const tfh_vm* appearance
   return TFH_OK;
 }
 
+size_t mlp_fwd_workspace(const tfh_mlp* p, int32_t precision);
+
+// The residuals the forward alone writes (no backward buffers): sample
+// points, distances, densities, step sizes, weights and transmittances,
+// top-k indices, per-ray terms, appearance features and colours, and the
+// MLP's weight tables of the forward's precision.
+static size_t carve_render_fwd(Carve& cv, const tfh_render_config* cfg, const tfh_mlp* mlp,
+                               int64_t F, int64_t R, RenderWs& ws) {
+  const int64_t N = cfg->density_samples_per_ray, k = cfg->appearance_samples_per_ray;
+  const int64_t M = R * k;
+  ws.d.pts = cv.take<float>(R * N * 3);
+  ws.d.ts = cv.take<float>(R * N);
+  ws.d.sigma = cv.take<float>(R * N);
+  ws.d.delta = cv.take<float>(R * N);
+  ws.d.w = cv.take<float>(R * N);
+  ws.d.e = cv.take<float>(R * N);
+  ws.d.idx = cv.take<int32_t>(k < N ? R * k : 0);
+  if (k >= N) ws.d.idx = nullptr;
+  ws.d.ray = cv.take<float>(R * 4);
+  ws.d.out = nullptr;
+  ws.feats = cv.take<float>(M * F);
+  ws.rgb = cv.take<float>(M * 3);
+  ws.gout = ws.dc = ws.dfeat = ws.mse_part = nullptr;
+  ws.mlp_ws_bytes = mlp ? mlp_fwd_workspace(mlp, cfg->mlp_precision) : 0;
+  ws.mlp_ws = cv.take<char>((int64_t)ws.mlp_ws_bytes);
+  return cv.off;
+}
+
+// tfh_render_fwd with its residuals in `scratch` (cap bytes, owned by the
+// caller of this function) instead of a workspace argument. Every kernel the
+// forward launches overwrites what it reads later, so the scratch needs no
+// clearing between calls.
+int32_t render_fwd_scratch(const tfh_vm* density, const tfh_vm* appearance, const tfh_mlp* mlp,
+                           const tfh_render_config* cfg, const float* aabb, const float* origins,
+                           const float* directions, const uint32_t* camera_indices,
+                           int64_t num_rays, const float* jitter, const float* base_ts,
+                           const float* base_steps, const float* gumbel, float* out,
+                           void* (*scratch_of_device)(size_t cap), size_t cap, const char* who,
+                           hipStream_t st) {
+  TFH_TRY(check_render_args(density, appearance, mlp, cfg, aabb, origins, directions, num_rays,
+                            jitter, base_ts, base_steps, who));
+  if (num_rays == 0) return TFH_OK;
+  TFH_REQUIRE(out != nullptr, "%s: null output", who);
+  const bool rgb = cfg->mode == TFH_MODE_RGB;
+  const int64_t F = rgb ? 3 * appearance->channels : 0;
+  {
+    Carve size(nullptr, 0);
+    RenderWs probe;
+    const size_t need = carve_render_fwd(size, cfg, rgb ? mlp : nullptr, F, num_rays, probe);
+    if (need > cap)
+      return fail(TFH_ERR_WORKSPACE,
+                  "%s: %lld rays need %zu bytes of scratch, more than its fixed %zu bytes; "
+                  "render fewer rays per call or use tfh_render_fwd with a workspace",
+                  who, (long long)num_rays, need, cap);
+  }
+  TFH_TRY(require_device());
+  void* scratch = scratch_of_device(cap);
+  if (scratch == nullptr) return TFH_ERR_LAUNCH;  // the message is already recorded
+  Carve cv(scratch, cap);
+  RenderWs ws;
+  carve_render_fwd(cv, cfg, rgb ? mlp : nullptr, F, num_rays, ws);
+  if (!cv.ok) return fail(TFH_ERR_WORKSPACE, "%s: scratch carve failed", who);
+  RayIn in = make_rayin(cfg, aabb, origins, directions, num_rays, jitter, base_ts, base_steps,
+                        gumbel);
+  return render_forward(density, appearance, mlp, cfg, in, camera_indices, out, nullptr, nullptr,
+                        ws, st);
+}
 // tfh_train_step; with hyper != null the Adam step reads its scalars from
 // device memory (see adam_launch) instead of lr_tensor, lr_mlp, decay and
 // adam_count.

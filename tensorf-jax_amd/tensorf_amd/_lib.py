@@ -18,6 +18,7 @@ TFH_OK = 0
 MODE_RGB, MODE_DIST_MEDIAN, MODE_DIST_MEAN, MODE_DIST_MEDIAN_AS_WRITTEN = 0, 1, 2, 3
 PREC_FP32, PREC_BF16 = 0, 1
 SAMPLING_SCHEDULE, SAMPLING_EDGES = 0, 1
+RENDER_NOALLOC_SCRATCH_BYTES = 128 << 20  # TFH_RENDER_NOALLOC_SCRATCH_BYTES
 
 
 class TfhError(RuntimeError):
@@ -116,6 +117,8 @@ _SIGS = {
                                c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
     "tfh_render_bwd": (c_i32, [P(Vm), P(Vm), P(Mlp), P(RenderConfig), c_p, c_p, c_p, c_p, c_i64,
                                c_p, P(VmGrad), P(VmGrad), P(MlpGrad), c_p, c_sz, c_p]),
+    "tfh_render_fwd_noalloc": (c_i32, [P(Vm), P(Vm), P(Mlp), P(RenderConfig), c_p, c_p, c_p,
+                                       c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p]),
     "tfh_adam_step": (c_i32, [c_p, c_p, c_p, c_p, c_i64, c_f, c_f, c_f, c_f, c_f, c_i64, c_i32,
                               c_p]),
     "tfh_train_step": (c_i32, [P(Vm), P(Vm), P(Mlp), P(RenderConfig), c_p, c_p, c_p, c_p, c_p,
