@@ -121,49 +121,72 @@ def test_ground_truth_renderer_closed_form():
 
 def test_proposal64_matches_uniform512_at_a_fraction_of_the_samples():
     """What the learned proposal gains over the method without it: 64 field
-    samples placed by the proposal (BF16 MLP, default histogram pad) reach the
-    PSNR of the reference's stratified 512-sample sampler (FP32 MLP) on the
-    training rays and on a held-out view, at a third of the time per step or
-    less. The field starts transparent (density shift -10, TensoRF's original)
-    so that geometry, not a view-dependent shell at the box boundary, is
-    learned."""
-    log("case: proposal-64 vs uniform-512, 800 steps")
+    samples placed by the proposal (rounds 128 -> 64, BF16 MLP, 16,384 rays a
+    step) against the reference's stratified 512-sample sampler (FP32 MLP,
+    4,096 rays a step), each at its benchmark batch, on the 8-blob scene (24
+    views at 128x128, VM 128^3, 3,000 steps, the field starting transparent with
+    density shift -10 as TensoRF's original code). Judged on three held-out
+    views: the proposal's PSNR at most 0.1 dB below uniform-512's (or above)
+    and its median-distance error no worse, at 5x the training rays/s or more.
+    Measured on one MI355X: +3.08 dB held out (depth error 0.0205 against
+    0.0337) at 6.91x; with the training seeds 4 and 5 instead of 3: +1.04 and
+    +0.72 dB at 6.86x and 6.90x. The proposal's PSNR on its training batches is
+    2.3-2.6 dB lower than uniform-512's in these runs: uniform-512 fits the 24
+    training views more closely and generalises less."""
+    log("case: proposal-64 vs uniform-512, 3000 steps")
     import time
     from tensorf_amd import cameras, proposal, render, scenes, training, utils
-    scene, c2ws, data = _small_dataset(views=24, res=96)
-    held = scenes.hemisphere_cameras(25, radius=4.0, seed=7)[3]
-    focal = scenes.focal_800() * 96 / 800.0
-    hrays = cameras.pinhole_rays(held, 96, 96, focal, 0)
-    hgt, _ = scenes.render_ground_truth(scene, hrays, 2048)
-    res, held_psnr, sec = {}, {}, {}
-    for name, pc, n, prec in (
-            ("uniform-512", None, 512, "fp32"),
+    views, res, g, steps = 24, 128, 128, 3000
+    scene = scenes.bounded_blob_scene(seed=0)
+    c2ws = scenes.hemisphere_cameras(views, radius=4.0, seed=1)
+    focal = scenes.focal_800() * res / 800.0
+    data = scenes.make_dataset(scene, c2ws, res, res, focal, gt_samples=2048)
+    spiral = scenes.hemisphere_cameras(views + 3, radius=4.0, seed=7)
+    held = [cameras.pinhole_rays(spiral[i], res, res, focal, 0) for i in (3, 9, 15)]
+    truth = [scenes.render_ground_truth(scene, r, 2048, with_depth=True) for r in held]
+    res_psnr, held_psnr, depth_err, rate = {}, {}, {}, {}
+    for name, pc, n, prec, r in (
+            ("uniform-512", None, 512, "fp32", 4096),
             ("proposal-64", proposal.ProposalConfig(samples_round0=128, samples_round1=64,
-                                                    samples_out=64, grid_dim=32,
-                                                    anneal_steps=200), 64, "bf16")):
-        cfg = _config(g=64, proposal=pc, n=n, steps=800, density_shift=-10.0)
-        cfg = dataclasses.replace(cfg, mlp_precision=prec)
-        st = training.TrainState.initialize(cfg, 64, prng_key=3, num_cameras=24)
-        batches = data.minibatches(2048, seed=1)
+                                                    samples_out=64, anneal_steps=500), 64,
+             "bf16", 16384)):
+        cfg = dataclasses.replace(_config(g=g, proposal=pc, n=n, steps=steps,
+                                          density_shift=-10.0), mlp_precision=prec,
+                                  minibatch_size=r)
+        st = training.TrainState.initialize(cfg, g, prng_key=3, num_cameras=views)
+        batches = data.minibatches(r, seed=3)
         mses = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for it in range(800):
+        for it in range(steps):
             st, logs = st.training_step(next(batches))
-            if it >= 750:
+            if it >= steps - 50:
                 mses.append(logs["mse"])
         torch.cuda.synchronize()
-        sec[name] = (time.perf_counter() - t0) / 800
-        res[name] = utils.psnr_from_mse(float(torch.stack(mses).mean().item()))
-        pred = torch.tensor(render.render_rays_batched(st.appearance_mlp, st.learnable_params,
-                                                       st.aabb, hrays, None, st.render_config(),
-                                                       batch_size=65536), device=hgt.device)
-        held_psnr[name] = utils.psnr_from_mse(float(((pred - hgt) ** 2).mean()))
-        log(f"  {name}: training psnr over the last 50 steps {res[name]:.2f} dB, held-out "
-            f"view {held_psnr[name]:.2f} dB, {1e3 * sec[name]:.2f} ms/step")
-    assert res["proposal-64"] >= res["uniform-512"] - 1.0
-    assert held_psnr["proposal-64"] >= held_psnr["uniform-512"] - 1.0
-    assert sec["proposal-64"] * 3.0 <= sec["uniform-512"]
+        rate[name] = r * steps / (time.perf_counter() - t0)
+        res_psnr[name] = utils.psnr_from_mse(float(torch.stack(mses).mean().item()))
+        ps, des = [], []
+        for rays, (gt, _, gt_depth) in zip(held, truth):
+            pred = torch.tensor(render.render_rays_batched(
+                st.appearance_mlp, st.learnable_params, st.aabb, rays, None,
+                st.render_config(), batch_size=65536), device=gt.device)
+            ps.append(utils.psnr_from_mse(float(((pred - gt) ** 2).mean())))
+            dep = render.render_rays_batched(
+                st.appearance_mlp, st.learnable_params, st.aabb, rays, None,
+                st.render_config(render.RenderMode.DIST_MEDIAN), batch_size=65536)
+            gd = gt_depth.cpu().numpy()
+            both = np.isfinite(gd) & np.isfinite(dep)
+            des.append(float(np.abs(dep[both] - gd[both]).mean()))
+        held_psnr[name], depth_err[name] = float(np.mean(ps)), float(np.mean(des))
+        log(f"  {name}: training psnr over the last 50 steps {res_psnr[name]:.2f} dB, held-out "
+            f"{held_psnr[name]:.2f} dB ({', '.join(f'{x:.2f}' for x in ps)}), median-distance "
+            f"error {depth_err[name]:.4f}, {rate[name]:.0f} train rays/s")
+    speedup = rate["proposal-64"] / rate["uniform-512"]
+    log(f"  proposal-64 - uniform-512: {held_psnr['proposal-64'] - held_psnr['uniform-512']:+.2f} "
+        f"dB held out at {speedup:.2f}x the rays/s")
+    assert held_psnr["proposal-64"] >= held_psnr["uniform-512"] - 0.1
+    assert depth_err["proposal-64"] <= depth_err["uniform-512"]
+    assert speedup >= 5.0
 
 
 @pytest.mark.parametrize("precision,film", [("fp32", False), ("bf16", False), ("fp32", True),
