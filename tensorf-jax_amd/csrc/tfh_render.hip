@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "tfh_prop.h"
+#include "tfh_scatter_sorted.h"
 
 namespace tfh {
 
@@ -500,6 +501,7 @@ struct RenderWs {
   float* dlcopy = nullptr;
   float* alcopy = nullptr;
   int dnlc = 1, anlc = 1;
+  SortWs sort;  // the sorted appearance scatter (RGB mode)
 };
 
 static int32_t check_cfg(const tfh_render_config* cfg, const char* who) {
@@ -545,6 +547,7 @@ static size_t carve_render(Carve& cv, const tfh_render_config* cfg, const tfh_ml
   // tables + partials (the backward's need is the larger)
   ws.mlp_ws_bytes = mlp ? mlp_bwd_workspace(mlp, M, cfg->mlp_precision) : 0;
   ws.mlp_ws = cv.take<char>((int64_t)ws.mlp_ws_bytes);
+  if (mlp) take_sort_ws(cv, M, ws.sort);
   return cv.off;
 }
 
@@ -684,12 +687,19 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
                        ws.mlp_ws, ws.mlp_ws_bytes, st, ws.act, ws.act ? ws.rgb : nullptr));
   RaySamples src{ws.d.pts, ws.d.idx, k, N};
   ProfScope ps("appearance_scatter", st);
-  if (!ws.alcopy)
+  const bool sorted = sorted_scatter_ok(*appearance, M, ws.sort);
+  if (!ws.alcopy) {
+    if (sorted)
+      return launch_vm_scatter_sorted(*appearance, src, M, ws.dfeat, *agrad, 1, ws.sort, st);
     return launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, *agrad, st);
+  }
   TFH_TRY(zero_copies(ws.alcopy, ws.anlc * aln, st));
   const tfh_vm_grad ag{agrad->plane, ws.alcopy};
-  TFH_TRY((launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, ag, st, 1,
-                                               ws.anlc)));
+  if (sorted)
+    TFH_TRY(launch_vm_scatter_sorted(*appearance, src, M, ws.dfeat, ag, ws.anlc, ws.sort, st));
+  else
+    TFH_TRY((launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, ag, st, 1,
+                                                 ws.anlc)));
   return sum_copies(ws.alcopy, ws.anlc, aln, agrad->line, st);
 }
 

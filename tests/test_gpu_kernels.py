@@ -161,8 +161,16 @@ def test_topk_bit_exact(lib, n, k):
     w2 = rng.uniform(0.01, 1, size=(50, n)).astype(np.float32)
     idx2 = render.select_appearance_samples(dev(w2), k, dev(g)).cpu().numpy()
     want2 = orc.topk_indices(w2, k, g)
-    # log(w)+g is formed in FP32 on both sides; allow differences only at exact key ties
-    assert (idx2 == want2).mean() > 0.999
+    # log(w) + g is formed in FP32 on both sides (logf and numpy's log may
+    # differ by an ulp): a row may differ only by indices whose keys sit within
+    # FP32 rounding of the row's selection threshold
+    keys = np.log(w2.astype(np.float64)) + g.astype(np.float64)[None, :]
+    bad = np.nonzero(np.any(idx2 != want2, axis=1))[0]
+    for r in bad:
+        thr = np.sort(keys[r])[::-1][k - 1]
+        diff = np.setxor1d(idx2[r], want2[r])
+        assert np.all(np.abs(keys[r][diff] - thr) <= 4e-7 * max(abs(thr), 1.0)), (r, diff)
+    log(f"  gumbel top-k: {len(bad)} of {len(w2)} rows differ, all at key ties")
 
 
 def _camera_layout(rng, ncam, m, layout):
