@@ -281,9 +281,12 @@ struct BwdIn {
 // wave then scatters 4 samples at a time.
 constexpr int kCompBwdWaves = 4;
 
+// With ds_out != null (one wave per ray) the kernel stops after the scans
+// and writes dL/ds of every sample there instead of scattering it.
 __global__ void __launch_bounds__(64 * kCompBwdWaves) k_composite_bwd(tfh_vm dvm, BwdIn in,
                                                                      float* __restrict__ dc,
-                                                                     tfh_vm_grad dgrad, int nlc) {
+                                                                     tfh_vm_grad dgrad, int nlc,
+                                                                     float* __restrict__ ds_out) {
   // nlc consecutive copies of the line gradient: ray r adds into copy r % nlc
   dgrad.line += (int64_t)(blockIdx.x % nlc) * 3 * dvm.grid_dim * dvm.channels;
   extern __shared__ float sm[];
@@ -382,6 +385,10 @@ __global__ void __launch_bounds__(64 * kCompBwdWaves) k_composite_bwd(tfh_vm dvm
     }
   }
   __syncthreads();
+  if (ds_out) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) ds_out[r * N + i] = s_ds[i];
+    return;
+  }
   // density scatter: every channel of every mode receives dL/ds_i
   const int grp = lane >> 4, l16 = lane & 15;
   const int G = (int)dvm.grid_dim, C = (int)dvm.channels;
@@ -501,7 +508,8 @@ struct RenderWs {
   float* dlcopy = nullptr;
   float* alcopy = nullptr;
   int dnlc = 1, anlc = 1;
-  SortWs sort;  // the sorted appearance scatter (RGB mode)
+  SortWs sort;           // the sorted scatters (RGB mode)
+  float* ds = nullptr;   // (R*N) dL/ds of every sample, for the sorted density scatter
 };
 
 static int32_t check_cfg(const tfh_render_config* cfg, const char* who) {
@@ -547,7 +555,10 @@ static size_t carve_render(Carve& cv, const tfh_render_config* cfg, const tfh_ml
   // tables + partials (the backward's need is the larger)
   ws.mlp_ws_bytes = mlp ? mlp_bwd_workspace(mlp, M, cfg->mlp_precision) : 0;
   ws.mlp_ws = cv.take<char>((int64_t)ws.mlp_ws_bytes);
-  if (mlp) take_sort_ws(cv, M, ws.sort);
+  if (mlp) {
+    ws.ds = cv.take<float>(R * N);
+    take_sort_ws(cv, R * N, ws.sort);  // pairs of every sample (k <= N)
+  }
   return cv.off;
 }
 
@@ -670,6 +681,9 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
   BwdIn bi{ws.d.pts, ws.d.sigma, ws.d.delta, ws.d.w, ws.d.e, ws.d.ray, ws.rgb, gout, ws.d.idx, N, k};
   const int64_t dln = 3 * density->grid_dim * density->channels;
   const int64_t aln = 3 * appearance->grid_dim * appearance->channels;
+  // every density sample, row r*N + i
+  const RaySamples dsrc{ws.d.pts, nullptr, N, N};
+  const bool dsorted = ws.ds && sorted_scatter_ok(*density, R * N, ws.sort);
   {
     ProfScope ps("composite_bwd_density_scatter", st);
     tfh_vm_grad dg = *dgrad;
@@ -677,10 +691,21 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
       TFH_TRY(zero_copies(ws.dlcopy, ws.dnlc * dln, st));
       dg.line = ws.dlcopy;
     }
-    hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)R), dim3(64 * kCompBwdWaves),
-                       5 * N * sizeof(float), st, *density, bi, ws.dc, dg,
-                       ws.dlcopy ? ws.dnlc : 1);
-    TFH_TRY(check_launch("render bwd: compositing"));
+    const int nlc = ws.dlcopy ? ws.dnlc : 1;
+    if (dsorted) {
+      // the scans write dL/ds; the scatter then runs on the pairs sorted by cell
+      hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)R), dim3(64), 5 * N * sizeof(float), st,
+                         *density, bi, ws.dc, dg, nlc, ws.ds);
+      TFH_TRY(check_launch("render bwd: compositing"));
+      TFH_TRY(sort_pairs((int)density->grid_dim, dsrc, R * N, ws.sort, "density_sort", st));
+      TFH_TRY(scatter_sorted_pairs(*density, dsrc, R * N, ScalarGrad{ws.ds}, dg, nlc, ws.sort,
+                                   "density_scatter_planes", "density_scatter_lines", st));
+    } else {
+      hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)R), dim3(64 * kCompBwdWaves),
+                         5 * N * sizeof(float), st, *density, bi, ws.dc, dg, nlc,
+                         (float*)nullptr);
+      TFH_TRY(check_launch("render bwd: compositing"));
+    }
     if (ws.dlcopy) TFH_TRY(sum_copies(ws.dlcopy, ws.dnlc, dln, dgrad->line, st));
   }
   TFH_TRY(mlp_bwd_rows(mlp, ws.feats, dirs, cams, M, k, ws.dc, ws.dfeat, mgrad, cfg->mlp_precision,
@@ -688,15 +713,25 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
   RaySamples src{ws.d.pts, ws.d.idx, k, N};
   ProfScope ps("appearance_scatter", st);
   const bool sorted = sorted_scatter_ok(*appearance, M, ws.sort);
+  if (sorted) {
+    // with every sample shaded the rows are the density scatter's, and on the
+    // same grid its sorted pairs serve again
+    const bool shared = dsorted && ws.d.idx == nullptr && appearance->grid_dim == density->grid_dim;
+    if (!shared)
+      TFH_TRY(sort_pairs((int)appearance->grid_dim, src, M, ws.sort, "appearance_sort", st));
+  }
+  const RowGrad gsrc{ws.dfeat, (int)appearance->channels};
   if (!ws.alcopy) {
     if (sorted)
-      return launch_vm_scatter_sorted(*appearance, src, M, ws.dfeat, *agrad, 1, ws.sort, st);
+      return scatter_sorted_pairs(*appearance, src, M, gsrc, *agrad, 1, ws.sort,
+                                  "appearance_scatter_planes", "appearance_scatter_lines", st);
     return launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, *agrad, st);
   }
   TFH_TRY(zero_copies(ws.alcopy, ws.anlc * aln, st));
   const tfh_vm_grad ag{agrad->plane, ws.alcopy};
   if (sorted)
-    TFH_TRY(launch_vm_scatter_sorted(*appearance, src, M, ws.dfeat, ag, ws.anlc, ws.sort, st));
+    TFH_TRY(scatter_sorted_pairs(*appearance, src, M, gsrc, ag, ws.anlc, ws.sort,
+                                 "appearance_scatter_planes", "appearance_scatter_lines", st));
   else
     TFH_TRY((launch_vm_scatter<RaySamples, true>(*appearance, src, M, ws.dfeat, ag, st, 1,
                                                  ws.anlc)));

@@ -46,6 +46,23 @@ __device__ __forceinline__ int plane_bin(int m, const Tap& a, const Tap& b, int 
   return (m * (g + 1) + tap_code(a, g)) * (g + 1) + tap_code(b, g);
 }
 
+// Upstream gradient of a row's feature (mode m, channel c): a row-major
+// (M, 3C) array (appearance features), or one scalar per row that every
+// channel of every mode receives (the density head: sigma = softplus(sum of
+// the 3C features + shift), so dL/dfeature = dL/ds for all of them).
+struct RowGrad {
+  const float* g;
+  int C;
+  __device__ __forceinline__ float at(int64_t row, int m, int c) const {
+    return g[row * 3 * C + m * C + c];
+  }
+};
+
+struct ScalarGrad {
+  const float* g;
+  __device__ __forceinline__ float at(int64_t row, int, int) const { return g[row]; }
+};
+
 struct SortWs {
   uint32_t* order = nullptr;  // (3 M) pairs row*4 + mode, sorted by bin
   int32_t* counts = nullptr;  // (bins) pairs per bin, then the running cursor
@@ -160,10 +177,10 @@ __global__ void __launch_bounds__(256) k_bin_place(Src src, int64_t M, int g,
 // registers and adds them when the bin changes and at the end.
 constexpr int kPairsPerGroup = 64;
 
-template <class Src>
+template <class Src, class GS>
 __global__ void __launch_bounds__(256) k_plane_scatter_sorted(tfh_vm vm, Src src, int64_t pairs,
                                                               const uint32_t* __restrict__ order,
-                                                              const float* __restrict__ gout,
+                                                              GS gsrc,
                                                               float* __restrict__ gplane) {
   constexpr int LPS = 16, NQ = 64 / LPS;
   const int G = (int)vm.grid_dim, C = (int)vm.channels;
@@ -220,13 +237,12 @@ __global__ void __launch_bounds__(256) k_plane_scatter_sorted(tfh_vm vm, Src src
       const float w[4] = {ta.w0 * tb.w0, ta.w0 * tb.w1, ta.w1 * tb.w0, ta.w1 * tb.w1};
       const float* lrow0 = vm.line + (size_t)(m * G + tv.i0) * C;
       const float* lrow1 = vm.line + (size_t)(m * G + tv.i1) * C;
-      const float* grow = gout + row * 3 * C + m * C;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int c = l + LPS * q;
         if (LPS * q < C && c < C) {
           const float lv = tv.w0 * lrow0[c] + tv.w1 * lrow1[c];
-          const float gl = grow[c] * lv;
+          const float gl = gsrc.at(row, m, c) * lv;
 #pragma unroll
           for (int t = 0; t < 4; ++t) acc[t][q] += w[t] * gl;
         }
@@ -240,9 +256,9 @@ __global__ void __launch_bounds__(256) k_plane_scatter_sorted(tfh_vm vm, Src src
 // gout[row][m*C + c]; 16-lane groups, equal line texels of the wave's four
 // rows (in either tap slot) added once. grad_line holds nlc copies; block b
 // adds into copy b % nlc.
-template <class Src>
+template <class Src, class GS>
 __global__ void __launch_bounds__(kGatherThreads) k_line_scatter(tfh_vm vm, Src src, int64_t M,
-                                                                 const float* __restrict__ gout,
+                                                                 GS gsrc,
                                                                  float* __restrict__ gline,
                                                                  int nlc) {
   constexpr int LPS = 16, SPB = kGatherThreads / LPS, NQ = 64 / LPS;
@@ -287,7 +303,7 @@ __global__ void __launch_bounds__(kGatherThreads) k_line_scatter(tfh_vm vm, Src 
         if (on) {
           float lv, pv;
           tap_values(vm.plane, vm.line, C, c, a, lv, pv);
-          const float gp = gout[s * 3 * C + m * C + c] * pv;
+          const float gp = gsrc.at(s, m, c) * pv;
           v0 = a.wl0 * gp;
           v1 = a.wl1 * gp;
           if (fold) {
@@ -331,42 +347,46 @@ inline bool sorted_scatter_ok(const tfh_vm& vm, int64_t M, const SortWs& s) {
          M < (1ll << 30);
 }
 
-// The full scatter of row-major gradients gout (M, 3C) of a field: planes
-// into grad.plane, lines into grad.line (nlc copies).
+// Sorts the (row, mode) pairs of M rows by plane cell of a grid of size g.
 template <class Src>
-inline int32_t launch_vm_scatter_sorted(const tfh_vm& vm, const Src& src, int64_t M,
-                                        const float* gout, const tfh_vm_grad& grad, int nlc,
-                                        const SortWs& s, hipStream_t st) {
-  if (M <= 0) return TFH_OK;
-  const int G = (int)vm.grid_dim;
-  const int64_t bins = 3ll * (G + 1) * (G + 1);
+inline int32_t sort_pairs(int g, const Src& src, int64_t M, const SortWs& s, const char* stage,
+                          hipStream_t st) {
+  const int64_t bins = 3ll * (g + 1) * (g + 1);
   const int64_t chunks = ceil_div(bins, kScanChunk);
   const int64_t rb = ceil_div(M, 256);
+  ProfScope ps(stage, st);
+  TFH_TRY(zero_copies(reinterpret_cast<float*>(s.counts), bins, st));  // all-zero bits
+  hipLaunchKernelGGL(k_bin_count<Src>, dim3((unsigned)rb), dim3(256), 0, st, src, M, g, s.counts);
+  hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)chunks), dim3(256), 0, st, s.counts, s.starts,
+                     bins, s.chunk);
+  hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, st, s.chunk, chunks);
+  hipLaunchKernelGGL(k_scan_finish, dim3((unsigned)ceil_div(bins, 256)), dim3(256), 0, st,
+                     s.counts, s.starts, bins, s.chunk);
+  hipLaunchKernelGGL(k_bin_place<Src>, dim3((unsigned)rb), dim3(256), 0, st, src, M, g, s.counts,
+                     s.order);
+  return check_launch("vm scatter: pair sort");
+}
+
+// The scatter of a field's feature gradients (gsrc) for M rows whose pairs
+// sort_pairs has ordered: planes into grad.plane, lines into grad.line (nlc
+// copies). stages: profile names of the plane and line passes.
+template <class Src, class GS>
+inline int32_t scatter_sorted_pairs(const tfh_vm& vm, const Src& src, int64_t M, const GS& gsrc,
+                                    const tfh_vm_grad& grad, int nlc, const SortWs& s,
+                                    const char* plane_stage, const char* line_stage,
+                                    hipStream_t st) {
+  if (M <= 0) return TFH_OK;
   {
-    ProfScope ps("appearance_sort", st);
-    TFH_TRY(zero_copies(reinterpret_cast<float*>(s.counts), bins, st));  // all-zero bits
-    hipLaunchKernelGGL(k_bin_count<Src>, dim3((unsigned)rb), dim3(256), 0, st, src, M, G,
-                       s.counts);
-    hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)chunks), dim3(256), 0, st, s.counts,
-                       s.starts, bins, s.chunk);
-    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, st, s.chunk, chunks);
-    hipLaunchKernelGGL(k_scan_finish, dim3((unsigned)ceil_div(bins, 256)), dim3(256), 0, st,
-                       s.counts, s.starts, bins, s.chunk);
-    hipLaunchKernelGGL(k_bin_place<Src>, dim3((unsigned)rb), dim3(256), 0, st, src, M, G,
-                       s.counts, s.order);
-    TFH_TRY(check_launch("vm scatter: pair sort"));
-  }
-  {
-    ProfScope ps("appearance_scatter_planes", st);
+    ProfScope ps(plane_stage, st);
     const int64_t pairs = 3 * M;
     const int64_t groups = ceil_div(pairs, kPairsPerGroup);
-    hipLaunchKernelGGL(k_plane_scatter_sorted<Src>, dim3((unsigned)ceil_div(groups, 16)),
-                       dim3(256), 0, st, vm, src, pairs, s.order, gout, grad.plane);
+    hipLaunchKernelGGL((k_plane_scatter_sorted<Src, GS>), dim3((unsigned)ceil_div(groups, 16)),
+                       dim3(256), 0, st, vm, src, pairs, s.order, gsrc, grad.plane);
     TFH_TRY(check_launch("vm scatter: planes"));
   }
-  ProfScope ps("appearance_scatter_lines", st);
-  hipLaunchKernelGGL(k_line_scatter<Src>, dim3((unsigned)ceil_div(M, kGatherThreads / 16)),
-                     dim3(kGatherThreads), 0, st, vm, src, M, gout, grad.line, nlc);
+  ProfScope ps(line_stage, st);
+  hipLaunchKernelGGL((k_line_scatter<Src, GS>), dim3((unsigned)ceil_div(M, kGatherThreads / 16)),
+                     dim3(kGatherThreads), 0, st, vm, src, M, gsrc, grad.line, nlc);
   return check_launch("vm scatter: lines");
 }
 
