@@ -16,11 +16,8 @@
 
 namespace tfh {
 
-// Grids up to this size use the sorted plane scatter (bins: 3 (G+1)^2).
+// Grids up to this size use the sorted plane scatter.
 constexpr int kSortMaxGrid = 512;
-constexpr int64_t kSortMaxBins = 3ll * (kSortMaxGrid + 1) * (kSortMaxGrid + 1);
-constexpr int kScanChunk = 4096;  // counts scanned by one workgroup
-constexpr int64_t kSortMaxChunks = (kSortMaxBins + kScanChunk - 1) / kScanChunk;
 
 // Code of one axis' tap pair: the lower texel when the pair is (i, i+1),
 // G-1 for the pair clamped at the low edge (0, 0), G for the high edge
@@ -63,110 +60,214 @@ struct ScalarGrad {
   __device__ __forceinline__ float at(int64_t row, int, int) const { return g[row]; }
 };
 
+// Two-level sort of the (row, mode) pairs by plane cell. Cells are grouped
+// into tiles of kTile x kTile cells per mode. (A) every workgroup ranks its
+// pairs within their tiles in LDS and reserves each tile's share with one
+// global atomic; (B) one workgroup scans the tile counts; (C) every pair
+// moves to its tile's segment; (D) every chunk of up to kSortChunk pairs of
+// one tile is counting-sorted by cell in LDS in place. Equal cells are then
+// adjacent within a chunk (a tile larger than a chunk repeats a cell in each
+// of its chunks). Global atomics: one per (workgroup, tile) instead of two
+// per pair.
+constexpr int kTile = 16;
+constexpr int kSortMaxTiles = 3 * ((kSortMaxGrid + 1 + kTile - 1) / kTile) *
+                              ((kSortMaxGrid + 1 + kTile - 1) / kTile);
+constexpr int kRowsPerRankGroup = 1024;  // rows of one workgroup of pass A
+constexpr int kSortChunk = 4096;         // pairs sorted by cell in one workgroup
+
 struct SortWs {
-  uint32_t* order = nullptr;  // (3 M) pairs row*4 + mode, sorted by bin
-  int32_t* counts = nullptr;  // (bins) pairs per bin, then the running cursor
-  int32_t* starts = nullptr;  // (bins) first pair of each bin
-  int32_t* chunk = nullptr;   // (chunks) pairs per scan chunk, then its offset
+  uint32_t* order = nullptr;    // (3 M) pairs row*4 + mode, sorted by cell
+  uint16_t* tile_of = nullptr;  // (3 M) tile of pair row*3 + mode
+  uint32_t* rank_of = nullptr;  // (3 M) position of the pair within its tile
+  int32_t* tile_count = nullptr;  // (tiles) pairs per tile
+  int32_t* tile_start = nullptr;  // (tiles + 1) first pair of each tile
+  int32_t* chunk_start = nullptr; // (tiles + 1) first chunk of each tile
 };
 
-// Pairs per bin. One thread per row: its three (row, mode) pairs.
-template <class Src>
-__global__ void __launch_bounds__(256) k_bin_count(Src src, int64_t M, int g,
-                                                   int32_t* __restrict__ counts) {
-  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (s >= M) return;
-  float x, y, z;
-  src.get(s, x, y, z);
-  ModeTaps mt[3];
-  mode_taps(x, y, z, g, mt);
-#pragma unroll
-  for (int m = 0; m < 3; ++m) atomicAdd(counts + plane_bin(m, mt[m].a, mt[m].b, g), 1);
+__device__ __forceinline__ int tiles_per_axis(int g) { return (g + 1 + kTile - 1) / kTile; }
+
+// Tile and cell (0..kTile^2-1) of mode m's plane taps.
+__device__ __forceinline__ void pair_tile(int m, const Tap& a, const Tap& b, int g, int& tile,
+                                          int& cell) {
+  const int ca = tap_code(a, g), cb = tap_code(b, g), nt = tiles_per_axis(g);
+  tile = (m * nt + ca / kTile) * nt + cb / kTile;
+  cell = (ca % kTile) * kTile + cb % kTile;
 }
 
-// Exclusive scan of the bin counts in chunks of kScanChunk: each workgroup
-// (256 threads, 16 counts each) scans its chunk in place and records the
-// chunk total.
-__global__ void __launch_bounds__(256) k_scan_chunks(int32_t* __restrict__ counts,
-                                                     int32_t* __restrict__ starts, int64_t bins,
-                                                     int32_t* __restrict__ chunk_total) {
+// (A) Per workgroup of kRowsPerRankGroup rows: LDS ranks within tiles, one
+// global reservation per tile present.
+template <class Src>
+__global__ void __launch_bounds__(256) k_tile_rank(Src src, int64_t M, int g,
+                                                   int32_t* __restrict__ tile_count,
+                                                   uint16_t* __restrict__ tile_of,
+                                                   uint32_t* __restrict__ rank_of) {
+  constexpr int RPT = kRowsPerRankGroup / 256;
+  __shared__ int32_t cnt[kSortMaxTiles];
+  const int nt = tiles_per_axis(g), tiles = 3 * nt * nt;
+  for (int t = threadIdx.x; t < tiles; t += 256) cnt[t] = 0;
+  __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerRankGroup;
+  int tl[RPT][3], rk[RPT][3];
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) {
+    const int64_t s = r0 + j * 256 + threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) tl[j][m] = -1;
+    if (s < M) {
+      float x, y, z;
+      src.get(s, x, y, z);
+      ModeTaps mt[3];
+      mode_taps(x, y, z, g, mt);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        int cell;
+        pair_tile(m, mt[m].a, mt[m].b, g, tl[j][m], cell);
+        rk[j][m] = atomicAdd(&cnt[tl[j][m]], 1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < tiles; t += 256) {
+    const int c = cnt[t];
+    if (c > 0) cnt[t] = atomicAdd(tile_count + t, c);  // this workgroup's base in the tile
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) {
+    const int64_t s = r0 + j * 256 + threadIdx.x;
+    if (s < M) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        tile_of[3 * s + m] = (uint16_t)tl[j][m];
+        rank_of[3 * s + m] = (uint32_t)(cnt[tl[j][m]] + rk[j][m]);
+      }
+    }
+  }
+}
+
+// (B) Exclusive scans of the tile counts and of the chunks per tile (one
+// workgroup; tiles <= kSortMaxTiles).
+__global__ void __launch_bounds__(1024) k_tile_scan(const int32_t* __restrict__ tile_count,
+                                                    int tiles, int32_t* __restrict__ tile_start,
+                                                    int32_t* __restrict__ chunk_start) {
+  __shared__ int32_t wsum[2][16];
+  __shared__ int32_t carry[2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+  __syncthreads();
+  for (int t0 = 0; t0 < tiles; t0 += 1024) {
+    const int t = t0 + threadIdx.x;
+    const int32_t c = t < tiles ? tile_count[t] : 0;
+    const int32_t ch = (c + kSortChunk - 1) / kSortChunk;
+    int32_t ic = c, ih = ch;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t a = __shfl_up(ic, o, 64), b = __shfl_up(ih, o, 64);
+      if (lane >= o) {
+        ic += a;
+        ih += b;
+      }
+    }
+    if (lane == 63) {
+      wsum[0][w] = ic;
+      wsum[1][w] = ih;
+    }
+    __syncthreads();
+    int32_t bc = carry[0], bh = carry[1];
+    for (int j = 0; j < w; ++j) {
+      bc += wsum[0][j];
+      bh += wsum[1][j];
+    }
+    if (t < tiles) {
+      tile_start[t] = bc + ic - c;
+      chunk_start[t] = bh + ih - ch;
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) {
+      carry[0] = bc + ic;
+      carry[1] = bh + ih;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    tile_start[tiles] = carry[0];
+    chunk_start[tiles] = carry[1];
+  }
+}
+
+// (C) Every pair to its tile's segment.
+__global__ void __launch_bounds__(256) k_tile_place(int64_t pairs,
+                                                    const uint16_t* __restrict__ tile_of,
+                                                    const uint32_t* __restrict__ rank_of,
+                                                    const int32_t* __restrict__ tile_start,
+                                                    uint32_t* __restrict__ order) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= pairs) return;
+  const int64_t row = p / 3;
+  const int m = (int)(p - 3 * row);
+  order[tile_start[tile_of[p]] + rank_of[p]] = (uint32_t)row * 4u + (uint32_t)m;
+}
+
+// (D) One chunk of a tile: counting sort by cell in LDS, written back in
+// place. Workgroup w finds its tile by binary search of chunk_start;
+// workgroups past the last chunk return.
+template <class Src>
+__global__ void __launch_bounds__(256) k_chunk_sort(Src src, int g, int tiles,
+                                                    const int32_t* __restrict__ tile_start,
+                                                    const int32_t* __restrict__ chunk_start,
+                                                    uint32_t* __restrict__ order) {
+  constexpr int PT = kSortChunk / 256, NC = kTile * kTile;
+  __shared__ int32_t hist[NC];
   __shared__ int32_t wsum[4];
-  const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * 16;
-  int32_t v[16];
-  int32_t run = 0;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    v[i] = (base + i < bins) ? counts[base + i] : 0;
-    run += v[i];
+  const int w = blockIdx.x;
+  if (w >= chunk_start[tiles]) return;
+  int lo = 0, hi = tiles - 1;  // last tile with chunk_start <= w
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (chunk_start[mid] <= w) lo = mid; else hi = mid - 1;
   }
-  // exclusive scan of the per-thread totals over the workgroup
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int32_t incl = run;
+  const int t = lo;
+  const int64_t base = (int64_t)tile_start[t] + (int64_t)(w - chunk_start[t]) * kSortChunk;
+  const int len = (int)min((int64_t)kSortChunk, (int64_t)tile_start[t + 1] - base);
+  for (int c = threadIdx.x; c < NC; c += 256) hist[c] = 0;
+  __syncthreads();
+  uint32_t pk[PT];
+  int cl[PT], rk[PT];
+#pragma unroll
+  for (int j = 0; j < PT; ++j) {
+    const int i = j * 256 + threadIdx.x;
+    cl[j] = -1;
+    if (i < len) {
+      pk[j] = order[base + i];
+      float x, y, z;
+      src.get((int64_t)(pk[j] >> 2), x, y, z);
+      Tap tv, ta, tb;
+      one_mode_taps(x, y, z, g, (int)(pk[j] & 3u), tv, ta, tb);
+      int tile;
+      pair_tile((int)(pk[j] & 3u), ta, tb, g, tile, cl[j]);
+      rk[j] = atomicAdd(&hist[cl[j]], 1);
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the 256 cell counts (one per thread)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int32_t c0 = hist[threadIdx.x];
+  int32_t inc = c0;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const int32_t t = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += t;
+    const int32_t a = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += a;
   }
-  if (lane == 63) wsum[w] = incl;
+  if (lane == 63) wsum[wv] = inc;
   __syncthreads();
   int32_t before = 0;
-  for (int i = 0; i < w; ++i) before += wsum[i];
-  int32_t off = before + incl - run;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    if (base + i < bins) starts[base + i] = off;
-    off += v[i];
-  }
-  if (threadIdx.x == 255) chunk_total[blockIdx.x] = before + incl;
-}
-
-// Chunk offsets (one workgroup), then every bin's start += its chunk's offset
-// and the cursor (counts) = start.
-__global__ void __launch_bounds__(1024) k_scan_offsets(int32_t* __restrict__ chunk_total,
-                                                       int64_t chunks) {
-  __shared__ int32_t wsum[16];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t i = threadIdx.x;
-  const int32_t v = i < chunks ? chunk_total[i] : 0;
-  int32_t incl = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int32_t t = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += t;
-  }
-  if (lane == 63) wsum[w] = incl;
+  for (int j = 0; j < wv; ++j) before += wsum[j];
   __syncthreads();
-  int32_t before = 0;
-  for (int j = 0; j < w; ++j) before += wsum[j];
-  if (i < chunks) chunk_total[i] = before + incl - v;
-}
-
-__global__ void __launch_bounds__(256) k_scan_finish(int32_t* __restrict__ counts,
-                                                     int32_t* __restrict__ starts, int64_t bins,
-                                                     const int32_t* __restrict__ chunk_off) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= bins) return;
-  const int32_t s = starts[i] + chunk_off[i / kScanChunk];
-  starts[i] = s;
-  counts[i] = s;
-}
-
-// Placement: pair (row, mode) goes to the next slot of its bin.
-template <class Src>
-__global__ void __launch_bounds__(256) k_bin_place(Src src, int64_t M, int g,
-                                                   int32_t* __restrict__ cursor,
-                                                   uint32_t* __restrict__ order) {
-  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (s >= M) return;
-  float x, y, z;
-  src.get(s, x, y, z);
-  ModeTaps mt[3];
-  mode_taps(x, y, z, g, mt);
+  hist[threadIdx.x] = before + inc - c0;
+  __syncthreads();
 #pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    const int32_t pos = atomicAdd(cursor + plane_bin(m, mt[m].a, mt[m].b, g), 1);
-    order[pos] = (uint32_t)s * 4u + (uint32_t)m;
-  }
+  for (int j = 0; j < PT; ++j)
+    if (cl[j] >= 0) order[base + hist[cl[j]] + rk[j]] = pk[j];
 }
 
 // Plane gradients from the sorted pairs: dL/dplane[m][texel][c] +=
@@ -326,17 +427,19 @@ __global__ void __launch_bounds__(kGatherThreads) k_line_scatter(tfh_vm vm, Src 
   }
 }
 
-// Bytes of the sort's workspace for M rows.
+// Bytes of the sort's workspace for up to M rows.
 inline size_t sort_ws_bytes(int64_t M) {
-  return align_up((size_t)(3 * M) * 4, 256) + 2 * align_up((size_t)kSortMaxBins * 4, 256) +
-         align_up((size_t)kSortMaxChunks * 4, 256);
+  return align_up((size_t)(3 * M) * 4, 256) * 2 + align_up((size_t)(3 * M) * 2, 256) +
+         3 * align_up((size_t)(kSortMaxTiles + 1) * 4, 256);
 }
 
 inline void take_sort_ws(Carve& cv, int64_t M, SortWs& s) {
   s.order = cv.take<uint32_t>(3 * M);
-  s.counts = cv.take<int32_t>(kSortMaxBins);
-  s.starts = cv.take<int32_t>(kSortMaxBins);
-  s.chunk = cv.take<int32_t>(kSortMaxChunks);
+  s.rank_of = cv.take<uint32_t>(3 * M);
+  s.tile_of = cv.take<uint16_t>(3 * M);
+  s.tile_count = cv.take<int32_t>(kSortMaxTiles + 1);
+  s.tile_start = cv.take<int32_t>(kSortMaxTiles + 1);
+  s.chunk_start = cv.take<int32_t>(kSortMaxTiles + 1);
 }
 
 // Whether a field takes the sorted scatter: 9..64 channels (16-lane
@@ -351,19 +454,20 @@ inline bool sorted_scatter_ok(const tfh_vm& vm, int64_t M, const SortWs& s) {
 template <class Src>
 inline int32_t sort_pairs(int g, const Src& src, int64_t M, const SortWs& s, const char* stage,
                           hipStream_t st) {
-  const int64_t bins = 3ll * (g + 1) * (g + 1);
-  const int64_t chunks = ceil_div(bins, kScanChunk);
-  const int64_t rb = ceil_div(M, 256);
+  const int nt = (g + 1 + kTile - 1) / kTile, tiles = 3 * nt * nt;
+  const int64_t pairs = 3 * M;
   ProfScope ps(stage, st);
-  TFH_TRY(zero_copies(reinterpret_cast<float*>(s.counts), bins, st));  // all-zero bits
-  hipLaunchKernelGGL(k_bin_count<Src>, dim3((unsigned)rb), dim3(256), 0, st, src, M, g, s.counts);
-  hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)chunks), dim3(256), 0, st, s.counts, s.starts,
-                     bins, s.chunk);
-  hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, st, s.chunk, chunks);
-  hipLaunchKernelGGL(k_scan_finish, dim3((unsigned)ceil_div(bins, 256)), dim3(256), 0, st,
-                     s.counts, s.starts, bins, s.chunk);
-  hipLaunchKernelGGL(k_bin_place<Src>, dim3((unsigned)rb), dim3(256), 0, st, src, M, g, s.counts,
-                     s.order);
+  TFH_TRY(zero_copies(reinterpret_cast<float*>(s.tile_count), tiles, st));  // all-zero bits
+  hipLaunchKernelGGL(k_tile_rank<Src>, dim3((unsigned)ceil_div(M, kRowsPerRankGroup)), dim3(256),
+                     0, st, src, M, g, s.tile_count, s.tile_of, s.rank_of);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, s.tile_count, tiles, s.tile_start,
+                     s.chunk_start);
+  hipLaunchKernelGGL(k_tile_place, dim3((unsigned)ceil_div(pairs, 256)), dim3(256), 0, st, pairs,
+                     s.tile_of, s.rank_of, s.tile_start, s.order);
+  // at most one partial chunk per tile beyond pairs / kSortChunk full ones
+  const int64_t chunk_bound = pairs / kSortChunk + tiles;
+  hipLaunchKernelGGL(k_chunk_sort<Src>, dim3((unsigned)chunk_bound), dim3(256), 0, st, src, g,
+                     tiles, s.tile_start, s.chunk_start, s.order);
   return check_launch("vm scatter: pair sort");
 }
 
