@@ -244,7 +244,14 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
 }
 
 // Weight-gradient tile: acc(i, j) = sum_k X[k][row0 + i] * Y[k][col0 + j],
-// k < K (the rows of the tile).
+// k < K (the rows of the tile). Each lane reads single floats down a column,
+// so the four lane groups of one read instruction hit the rows of four
+// different k; the k order is permuted so that those rows are 4 apart
+// (FP32: k = 16m + 4kq + c for MFMA c of chunk m; BF16: element e of lane
+// group kq is k = 32m + 16(e/4) + 4kq + e%4). With row strides of 4 mod 8
+// floats, 4 rows are 16 mod 32 floats apart: the four groups' 16-float
+// segments fall in disjoint banks. The sum over k is unchanged; only the
+// order of the FP32 accumulation differs. Requires K % 16 == 0 (32 for BF16).
 template <bool BF, int K>
 __device__ __forceinline__ void mma_wgrad_acc(const float* __restrict__ X, int ldx, int row0,
                                               const float* __restrict__ Y, int ldy, int col0,
@@ -252,22 +259,29 @@ __device__ __forceinline__ void mma_wgrad_acc(const float* __restrict__ X, int l
   const int lane = lane_id();
   const int r = lane & 15, kq = lane >> 4;
   if constexpr (BF) {
+    static_assert(K % 32 == 0, "BF16 weight-gradient tiles take K in steps of 32");
 #pragma unroll
     for (int k0 = 0; k0 < K; k0 += 32) {
       bf16x8 av, bv;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        av[e] = (__bf16)X[(k0 + 8 * kq + e) * ldx + row0 + r];
-        bv[e] = (__bf16)Y[(k0 + 8 * kq + e) * ldy + col0 + r];
+        const int k = k0 + 16 * (e >> 2) + 4 * kq + (e & 3);
+        av[e] = (__bf16)X[k * ldx + row0 + r];
+        bv[e] = (__bf16)Y[k * ldy + col0 + r];
       }
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
     }
   } else {
+    static_assert(K % 16 == 0, "FP32 weight-gradient tiles take K in steps of 16");
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 4) {
-      const float a = X[(k0 + kq) * ldx + row0 + r];
-      const float b = Y[(k0 + kq) * ldy + col0 + r];
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int k = k0 + 4 * kq + c;
+        const float a = X[k * ldx + row0 + r];
+        const float b = Y[k * ldy + col0 + r];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+      }
     }
   }
 }

@@ -756,7 +756,13 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   f32x4 wacc[NT > 0 ? NT : 1];
 #pragma unroll
   for (int i = 0; i < (NT > 0 ? NT : 1); ++i) wacc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float db1 = 0.f, db2 = 0.f, db3 = 0.f;  // bias gradients of column threadIdx.x
+  // Bias gradients, summed in registers: db1 / db2 per lane over the rows
+  // 4(lane/16)+e of the column 16*wave + lane%16 (the strip phases that
+  // produce dZ1 and dZ2 give wave w column tile w on every tile), db3 per
+  // thread t < 3*TM over its (row t/3, colour t%3) slot; reduced at the end.
+  static_assert((kU / 16) * (RT / (NW >= 16 ? 1 : RT)) == NW,
+                "one strip item per wave: the bias sums stay with their column");
+  float db1 = 0.f, db2 = 0.f, db3 = 0.f;
   if (NT == 0)
     for (int64_t e = threadIdx.x; e < gp.total; e += blockDim.x) part[e] = 0.0f;
   else
@@ -845,8 +851,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     // dW3 (U x 3) += H2^T dZ3 ; db3
     wgrad_phase<TM, NW, BF, NT, kLSlot3, kCnt3, true>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3, d.ld3,
                                                       part, gp.w3, kU, 3, Wl);
-    if (threadIdx.x < 3)
-      for (int i = 0; i < TM; ++i) db3 += Z3[i * d.ld3 + threadIdx.x];
+    if (threadIdx.x < 3 * TM) db3 += Z3[(threadIdx.x / 3) * d.ld3 + threadIdx.x % 3];
     __syncthreads();
     // Strip phases below run one work item per (column tile, row group).
     // dH2 = dZ3 @ W3^T; FiLM backward; dZ2 = dH2 * (h2 > 0) written over H2
@@ -879,7 +884,9 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
             }
             g = g * w.embed[(size_t)Cs[row] * kU + col - kU / 2];
           }
-          H2[row * d.ldU + col] = hp > 0.0f ? g : 0.0f;
+          const float dz = hp > 0.0f ? g : 0.0f;
+          H2[row * d.ldU + col] = dz;
+          db2 += dz;
         }
         if (film && tcam >= 0) {
           sh += __shfl_xor(sh, 16, 64);
@@ -897,7 +904,9 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
           const int rt = q / 4, e = q % 4;
           const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
           const float hp = H2p[row * d.ldU + col];
-          H2[row * d.ldU + col] = hp > 0.0f ? acc[rt][e] : 0.0f;
+          const float dz = hp > 0.0f ? acc[rt][e] : 0.0f;
+          H2[row * d.ldU + col] = dz;
+          db2 += dz;
         }
       }
     }
@@ -905,8 +914,6 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     // dW2 (U x U) += H1^T dZ2 ; db2
     wgrad_phase<TM, NW, BF, NT, kSlot2, kCnt2>(wacc, wave, wt.n2, kU / 16, H1, d.ldU, H2,
                                                d.ldU, part, gp.w2, kU, kU);
-    if (threadIdx.x < kU)
-      for (int i = 0; i < TM; ++i) db2 += H2[i * d.ldU + threadIdx.x];
     __syncthreads();
     // dH1 = dZ2 @ W2^T ; dZ1 = dH1 * (h1 > 0) written over H1 (dW2 is done)
     for (int it = wave; it < (kU / 16) * RG; it += NW) {
@@ -920,15 +927,15 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
         const int rt = q / 4, e = q % 4;
         const int row = r0 + 16 * rt + 4 * (lane >> 4) + e;
         const float h = H1[row * d.ldU + col];
-        H1[row * d.ldU + col] = h > 0.0f ? acc[rt][e] : 0.0f;
+        const float dz = h > 0.0f ? acc[rt][e] : 0.0f;
+        H1[row * d.ldU + col] = dz;
+        db1 += dz;
       }
     }
     __syncthreads();
     // dW1 (E x U) += X^T dZ1 ; db1
     wgrad_phase<TM, NW, BF, NT, kSlot1, kCnt1>(wacc, wave, wt.n1, kU / 16, X, d.ldE, H1,
                                                d.ldU, part, gp.w1, d.E, kU);
-    if (threadIdx.x < kU)
-      for (int i = 0; i < TM; ++i) db1 += H1[i * d.ldU + threadIdx.x];
     __syncthreads();
     // dX = dZ1 @ W1^T for the columns that feed h0: [0, S) and the FE(h0)
     // block, i.e. the column tiles below fe1 (the 3 direction columns between
@@ -1009,6 +1016,18 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
       for (int c = threadIdx.x; c < kU; c += blockDim.x)
         unsafeAtomicAdd(gembed + (size_t)rc * kU + c, Eacc[c]);
   }
+  // bias sums: db1 / db2 over the lane groups, db3 over the threads of a
+  // colour (in Ds: every wave is past its last tile after the barrier)
+  db1 += __shfl_xor(db1, 16, 64);
+  db1 += __shfl_xor(db1, 32, 64);
+  db2 += __shfl_xor(db2, 16, 64);
+  db2 += __shfl_xor(db2, 32, 64);
+  __syncthreads();
+  if (threadIdx.x < 3) Ds[threadIdx.x] = 0.0f;
+  __syncthreads();
+  if (threadIdx.x < 3 * TM) atomicAdd(Ds + threadIdx.x % 3, db3);
+  __syncthreads();
+  const int bcol = 16 * wave + lane;  // bias column of lanes < 16
   // the workgroup's gradient sums -> its partial
   if constexpr (NT > 0) {
 #pragma unroll
@@ -1035,18 +1054,17 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
         if (t < wt.n0) store_tile(part, gp.w0, d.F, d.S, 16 * (t / 2), 16 * (t % 2), v);
       }
     }
-    if (threadIdx.x < kU) {
-      part[gp.b1 + threadIdx.x] = db1;
-      part[gp.b2 + threadIdx.x] = db2;
+    if (lane < 16) {
+      part[gp.b1 + bcol] = db1;
+      part[gp.b2 + bcol] = db2;
     }
-    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] = db3;
+    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] = Ds[threadIdx.x];
   } else {
-    __syncthreads();
-    if (threadIdx.x < kU) {
-      part[gp.b1 + threadIdx.x] += db1;
-      part[gp.b2 + threadIdx.x] += db2;
+    if (lane < 16) {
+      part[gp.b1 + bcol] += db1;
+      part[gp.b2 + bcol] += db2;
     }
-    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] += db3;
+    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] += Ds[threadIdx.x];
   }
 }
 
