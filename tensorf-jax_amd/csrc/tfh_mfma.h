@@ -13,6 +13,18 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 
+// The lane index behind an empty volatile asm: each call computes it afresh,
+// so operand addresses derived from it are computed where a tile product
+// runs. With the plain index, loop-invariant code motion hoists every
+// product's addresses out of a persistent kernel's tile loop, where a kernel
+// at its register limit spills them and reloads each one (a scratch round
+// trip) in front of its MFMA.
+__device__ __forceinline__ int lane_here() {
+  int l = (int)(threadIdx.x & 63);
+  asm volatile("" : "+v"(l));
+  return l;
+}
+
 template <int NR>
 __device__ __forceinline__ void zero_acc(f32x4 (&acc)[NR]) {
 #pragma unroll
@@ -125,7 +137,7 @@ struct PackedB32 {
 template <int NR, int KMAX, class B4>
 __device__ __forceinline__ void mma_strip_f32(const float* __restrict__ A, int lda, int K, B4 b,
                                               int col0, f32x4 (&acc)[NR]) {
-  const int lane = lane_id();
+  const int lane = lane_here();
   const int r = lane & 15, kq = lane >> 4;
   constexpr int NS = KMAX / 16;
   for (int kc = 0; kc < K; kc += KMAX) {  // one pass for K <= KMAX
@@ -185,7 +197,7 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
 template <int NR, int KMAX, class BF32>
 __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, BF32 b,
                                           int col0, f32x4 (&acc)[NR]) {
-  const int lane = lane_id();
+  const int lane = lane_here();
   const int r = lane & 15, kq = lane >> 4;
   constexpr int NS = KMAX / 4;
   for (int kc = 0; kc < K; kc += KMAX) {  // one pass for K <= KMAX
@@ -208,7 +220,7 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
 template <int NR, int KMAX, class B16>
 __device__ __forceinline__ void mma_strip_b16(const float* __restrict__ A, int lda, int K, B16 b,
                                               int col0, f32x4 (&acc)[NR]) {
-  const int lane = lane_id();
+  const int lane = lane_here();
   const int r = lane & 15, kq = lane >> 4;
   constexpr int NS = (KMAX + 31) / 32;
   for (int kc = 0; kc < K; kc += 32 * NS) {  // one pass for K <= KMAX
@@ -252,11 +264,14 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
 // floats, 4 rows are 16 mod 32 floats apart: the four groups' 16-float
 // segments fall in disjoint banks. The sum over k is unchanged; only the
 // order of the FP32 accumulation differs. Requires K % 16 == 0 (32 for BF16).
-template <bool BF, int K>
+// FRESH: the lane index from lane_here (operand addresses computed per call)
+// or the plain one (addresses loop-invariant: hoisted out of a tile loop and
+// kept in registers when the kernel has them to spare).
+template <bool BF, int K, bool FRESH = true>
 __device__ __forceinline__ void mma_wgrad_acc(const float* __restrict__ X, int ldx, int row0,
                                               const float* __restrict__ Y, int ldy, int col0,
                                               f32x4& acc) {
-  const int lane = lane_id();
+  const int lane = FRESH ? lane_here() : lane_id();
   const int r = lane & 15, kq = lane >> 4;
   if constexpr (BF) {
     static_assert(K % 32 == 0, "BF16 weight-gradient tiles take K in steps of 32");

@@ -202,7 +202,7 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   }();
   const bool sd = is_std_dims(d, bf) && !force_partial;
   const size_t lds = bwd_lds_floats<kBwdTM>(d, sd ? kLdsSlots * kBwdNW : 0) * sizeof(float) +
-                     (sd && d.ncam == 0 ? (bf ? kG1LdsBytes : kG1LdsBytesF) : 0);
+                     (sd && d.ncam == 0 && kBwdNW == 8 ? (bf ? kG1LdsBytes : kG1LdsBytesF) : 0);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp bwd: tile needs %zu bytes of LDS", lds);
   {
     ProfScope ps("mlp_bwd", st);

@@ -664,7 +664,7 @@ __device__ __forceinline__ void wgrad_phase(f32x4 (&wacc)[NT > 0 ? NT : 1], int 
       const int t = wave + NW * si;
       if (t < n) {
         const int ti = t / tcols, tj = t % tcols;
-        mma_wgrad_acc<BF, TM>(Xm, ldx, 16 * ti, Ym, ldy, 16 * tj, wacc[SLOT0 + si]);
+        mma_wgrad_acc<BF, TM, false>(Xm, ldx, 16 * ti, Ym, ldy, 16 * tj, wacc[SLOT0 + si]);
       }
       // keep the operand loads of later tiles from being hoisted (they would
       // all be live at once)
@@ -745,7 +745,10 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   float* Wl = reinterpret_cast<float*>(Cs + 4 * TM) + (d.ncam > 0 ? kU + 4 : 0);
   const Operands<BF> op{d, w, pk};
 
-  const int wave = threadIdx.x >> 6;
+  // masked to the block's waves (NW, a power of two): lets the compiler fold
+  // each weight-gradient slot's tile offsets into constants
+  static_assert((NW & (NW - 1)) == 0, "wave count a power of two");
+  const int wave = (threadIdx.x >> 6) & (NW - 1);
   const int lane = lane_id();
   float* part = partials + (int64_t)blockIdx.x * gp.total;
   if (FILM && d.ncam > 0) {
@@ -757,9 +760,10 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
 #pragma unroll
   for (int i = 0; i < (NT > 0 ? NT : 1); ++i) wacc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
   // Bias gradients, summed in registers: db1 / db2 per lane over the rows
-  // 4(lane/16)+e of the column 16*wave + lane%16 (the strip phases that
-  // produce dZ1 and dZ2 give wave w column tile w on every tile), db3 per
-  // thread t < 3*TM over its (row t/3, colour t%3) slot; reduced at the end.
+  // 4(lane/16)+e of the row groups of column 16*(wave/RG) + lane%16 (the
+  // strip phases that produce dZ1 and dZ2 give wave w the same item on every
+  // tile), db3 per thread t < 3*TM over its (row t/3, colour t%3) slot;
+  // reduced at the end.
   static_assert((kU / 16) * (RT / (NW >= 16 ? 1 : RT)) == NW,
                 "one strip item per wave: the bias sums stay with their column");
   float db1 = 0.f, db2 = 0.f, db3 = 0.f;
@@ -767,8 +771,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     for (int64_t e = threadIdx.x; e < gp.total; e += blockDim.x) part[e] = 0.0f;
   else
     for (int e = threadIdx.x; e < kLdsSlots * NW * 256; e += blockDim.x) Wl[e] = 0.0f;
-  constexpr bool kG1Lds = BF && STD && !FILM && NT > 0;
-  constexpr bool kG1LdsF = !BF && STD && !FILM && NT > 0;
+  constexpr bool kG1Lds = BF && STD && !FILM && NT > 0 && NW == 8;
+  constexpr bool kG1LdsF = !BF && STD && !FILM && NT > 0 && NW == 8;
   __bf16* G1s = reinterpret_cast<__bf16*>(Wl + kLdsSlots * NW * 256);
   float* G1f = Wl + kLdsSlots * NW * 256;
   if constexpr (kG1Lds) {
@@ -1016,18 +1020,24 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
       for (int c = threadIdx.x; c < kU; c += blockDim.x)
         unsafeAtomicAdd(gembed + (size_t)rc * kU + c, Eacc[c]);
   }
-  // bias sums: db1 / db2 over the lane groups, db3 over the threads of a
-  // colour (in Ds: every wave is past its last tile after the barrier)
+  // bias sums: db1 / db2 over the lane groups, then over the waves of a
+  // column tile and db3 over the threads of a colour, in X (every wave is
+  // past its last tile after the barrier): [db1 | db2 | db3]
   db1 += __shfl_xor(db1, 16, 64);
   db1 += __shfl_xor(db1, 32, 64);
   db2 += __shfl_xor(db2, 16, 64);
   db2 += __shfl_xor(db2, 32, 64);
+  float* bsum = X;
   __syncthreads();
-  if (threadIdx.x < 3) Ds[threadIdx.x] = 0.0f;
+  for (int e = threadIdx.x; e < 2 * kU + 3; e += blockDim.x) bsum[e] = 0.0f;
   __syncthreads();
-  if (threadIdx.x < 3 * TM) atomicAdd(Ds + threadIdx.x % 3, db3);
+  if (lane < 16) {
+    const int bcol = 16 * (wave / RG) + lane;
+    atomicAdd(bsum + bcol, db1);
+    atomicAdd(bsum + kU + bcol, db2);
+  }
+  if (threadIdx.x < 3 * TM) atomicAdd(bsum + 2 * kU + threadIdx.x % 3, db3);
   __syncthreads();
-  const int bcol = 16 * wave + lane;  // bias column of lanes < 16
   // the workgroup's gradient sums -> its partial
   if constexpr (NT > 0) {
 #pragma unroll
@@ -1054,17 +1064,17 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
         if (t < wt.n0) store_tile(part, gp.w0, d.F, d.S, 16 * (t / 2), 16 * (t % 2), v);
       }
     }
-    if (lane < 16) {
-      part[gp.b1 + bcol] = db1;
-      part[gp.b2 + bcol] = db2;
+    if (threadIdx.x < kU) {
+      part[gp.b1 + threadIdx.x] = bsum[threadIdx.x];
+      part[gp.b2 + threadIdx.x] = bsum[kU + threadIdx.x];
     }
-    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] = Ds[threadIdx.x];
+    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] = bsum[2 * kU + threadIdx.x];
   } else {
-    if (lane < 16) {
-      part[gp.b1 + bcol] += db1;
-      part[gp.b2 + bcol] += db2;
+    if (threadIdx.x < kU) {
+      part[gp.b1 + threadIdx.x] += bsum[threadIdx.x];
+      part[gp.b2 + threadIdx.x] += bsum[kU + threadIdx.x];
     }
-    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] += Ds[threadIdx.x];
+    if (threadIdx.x < 3) part[gp.b3 + threadIdx.x] += bsum[2 * kU + threadIdx.x];
   }
 }
 
