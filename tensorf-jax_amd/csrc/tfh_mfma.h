@@ -301,6 +301,50 @@ __device__ __forceinline__ void mma_wgrad_acc(const float* __restrict__ X, int l
   }
 }
 
+// The same tile product split in two, for a software pipeline over tiles:
+// wgrad_load reads a lane's K/4 operands of X and of Y (in the permuted k
+// order of mma_wgrad_acc), wgrad_mma multiplies them into acc.
+template <bool BF, int K>
+__device__ __forceinline__ void wgrad_load(const float* __restrict__ X, int ldx, int row0,
+                                           const float* __restrict__ Y, int ldy, int col0,
+                                           int lane, float (&a)[K / 4], float (&b)[K / 4]) {
+  static_assert(K % (BF ? 32 : 16) == 0, "weight-gradient K in whole MFMA steps");
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int m = 0; m < K / 4; ++m) {
+    int k;
+    if constexpr (BF) {
+      const int e = m & 7;
+      k = 32 * (m >> 3) + 16 * (e >> 2) + 4 * kq + (e & 3);
+    } else {
+      k = 16 * (m >> 2) + 4 * kq + (m & 3);
+    }
+    a[m] = X[k * ldx + row0 + r];
+    b[m] = Y[k * ldy + col0 + r];
+  }
+}
+
+template <bool BF, int K>
+__device__ __forceinline__ void wgrad_mma(const float (&a)[K / 4], const float (&b)[K / 4],
+                                          f32x4& acc) {
+  if constexpr (BF) {
+#pragma unroll
+    for (int g = 0; g < K / 32; ++g) {
+      bf16x8 av, bv;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        av[e] = (__bf16)a[8 * g + e];
+        bv[e] = (__bf16)b[8 * g + e];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < K / 4; ++m)
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], b[m], acc, 0, 0, 0);
+  }
+}
+
 template <bool BF, int K>
 __device__ __forceinline__ f32x4 mma_wgrad(const float* __restrict__ X, int ldx, int row0,
                                            const float* __restrict__ Y, int ldy, int col0) {

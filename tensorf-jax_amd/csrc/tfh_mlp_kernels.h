@@ -659,15 +659,29 @@ __device__ __forceinline__ void wgrad_phase(f32x4 (&wacc)[NT > 0 ? NT : 1], int 
       __builtin_amdgcn_sched_barrier(0);
     }
   } else if constexpr (NT > 0) {
+    // two-stage pipeline over the wave's tiles: the LDS reads of tile si + 1
+    // are issued before the products of tile si, so their latency overlaps
+    // the MFMAs (two operand sets live at a time)
+    const int lane = lane_id();
+    float a0[TM / 4], b0[TM / 4], a1[TM / 4], b1[TM / 4];
+    if (wave < n) wgrad_load<BF, TM>(Xm, ldx, 16 * (wave / tcols), Ym, ldy, 16 * (wave % tcols),
+                                     lane, a0, b0);
 #pragma unroll
     for (int si = 0; si < CNT; ++si) {
-      const int t = wave + NW * si;
-      if (t < n) {
-        const int ti = t / tcols, tj = t % tcols;
-        mma_wgrad_acc<BF, TM, false>(Xm, ldx, 16 * ti, Ym, ldy, 16 * tj, wacc[SLOT0 + si]);
+      const int t = wave + NW * si, tn = t + NW;
+      if (si + 1 < CNT && tn < n) {
+        if (si & 1)
+          wgrad_load<BF, TM>(Xm, ldx, 16 * (tn / tcols), Ym, ldy, 16 * (tn % tcols), lane, a0, b0);
+        else
+          wgrad_load<BF, TM>(Xm, ldx, 16 * (tn / tcols), Ym, ldy, 16 * (tn % tcols), lane, a1, b1);
       }
-      // keep the operand loads of later tiles from being hoisted (they would
-      // all be live at once)
+      if (t < n) {
+        if (si & 1)
+          wgrad_mma<BF, TM>(a1, b1, wacc[SLOT0 + si]);
+        else
+          wgrad_mma<BF, TM>(a0, b0, wacc[SLOT0 + si]);
+      }
+      // keeps the reads of tile si + 2 from being hoisted above these products
       __builtin_amdgcn_sched_barrier(0);
     }
   } else {
