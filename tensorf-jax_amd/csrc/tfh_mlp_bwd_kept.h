@@ -7,17 +7,18 @@
 // of dW1 / dW2 stay in registers of their owning wave and those of dW3 / dW0
 // in LDS slots for the whole kernel (as in k_mlp_bwd). Per tile:
 //   commit  the tile's inputs, loaded into registers during the previous
-//           tile's last phase, go to LDS; dZ3 = grgb * s * (1 - s)
+//           tile's dW2 phase, go to LDS; dZ3 = grgb * s * (1 - s)
 //   T1      layer 0 as 8 items (2 column tiles x 2 row tiles x 2 K parts,
 //           one per wave; the parts land in X[:, 0:S] and H0) ; dW3 ; db3
 //   T2      encoding (h0 = the two parts' sum; sin / cos of the first
 //           frequency by sincosf, the doubled one by the double-angle
 //           identities) ; dZ2 = (dZ3 @ W3^T) * (h2 > 0) over H2 ; db2
-//   T3      dW2
+//   T3      the next tile's loads are issued (dW2 reads LDS only, so no
+//           wait of this phase queues behind them; in-order vmcnt) ; dW2
 //   T4      dZ1 = (dZ2 @ W2^T) * (h1 > 0) over H1 ; db1
 //   T5      dW1 ; dX = dZ1 @ W1^T (the columns that feed h0) over H2 / Z3
 //   T6      dH0 from dX and the sin / cos the encoding left in X
-//   T7      the next tile's loads are issued ; dW0 ; dF = dH0 @ W0^T
+//   T7      dW0 ; dF = dH0 @ W0^T
 // with one barrier after each step: 8 per tile.
 #pragma once
 
@@ -276,7 +277,8 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
       }
     }
     __syncthreads();
-    // T3: dW2 += H1^T dZ2
+    // T3: the next tile's loads (nothing in this phase waits on vmcnt) ;
+    // dW2 += H1^T dZ2
     kept_issue<TM, NT>(d, feats, dirs, act, rgb_in, grgb, row0 + (int64_t)gridDim.x * TM, M, rpr,
                        nx);
     wgrad_phase<TM, NW, BF, kSlots, kSlot2, kCnt2>(wacc, wave, wt.n2, kU / 16, H1, d.ldU, H2,
@@ -352,7 +354,7 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
       }
     }
     __syncthreads();
-    // T7: the next tile's loads ; dW0 += Fs^T dH0 ; dF = dH0 @ W0^T -> gfeat
+    // T7: dW0 += Fs^T dH0 ; dF = dH0 @ W0^T -> gfeat
     wgrad_phase<TM, NW, BF, kSlots, kLSlot0, kCnt0, true>(wacc, wave, wt.n0, 2, Fs, d.ldF, H0,
                                                           d.ldS, part, gp.w0, d.F, d.S, Wl);
     for (int it = wave; it < (d.FP / 16) * RT; it += NW) {
