@@ -29,12 +29,14 @@ namespace tfh {
 constexpr int kKeptTM = 32, kKeptNW = 8;
 constexpr int kKeptLdDX = 148;  // row stride of dX (144 columns)
 
-// The tile's inputs in registers between issue and commit.
-template <int TM, int NT>
+// The tile's inputs in registers between issue and commit (kept activations
+// in BF16 with the BF16 MLP).
+template <int TM, int NT, bool BF>
 struct KeptRegs {
   typedef int i32x4 __attribute__((ext_vector_type(4)));
   static constexpr int kF = (TM * 160 / 4 + NT - 1) / NT;  // float4 of features per thread
-  static constexpr int kA = TM * 64 / NT;                  // float4 of activations per thread
+  static constexpr int kA = TM * (BF ? 32 : 64) / NT;      // 16-byte activation loads per thread
+  static_assert(TM * (BF ? 32 : 64) % NT == 0, "activation loads fill the block");
   i32x4 vf[kF];
   i32x4 va[kA];
   float s, g, dv;
@@ -45,13 +47,13 @@ struct KeptRegs {
 // resource over the tile's valid rows, so rows past M read 0 (all of them
 // for a tile past the end: the loop issues one unconditionally, which keeps
 // the registers of r dead between commit and the next issue).
-template <int TM, int NT>
+template <int TM, int NT, bool BF>
 __device__ __forceinline__ void kept_issue(const MlpDims& d, const float* __restrict__ feats,
                                            const float* __restrict__ dirs,
                                            const float* __restrict__ act,
                                            const float* __restrict__ rgb,
                                            const float* __restrict__ grgb, int64_t row0,
-                                           int64_t M, int64_t rpr, KeptRegs<TM, NT>& r) {
+                                           int64_t M, int64_t rpr, KeptRegs<TM, NT, BF>& r) {
   const int nv = (int)max((int64_t)0, min((int64_t)TM, M - row0));
   const int64_t ray0 = row0 / rpr;
   const unsigned rem0 = (unsigned)(row0 - ray0 * rpr);
@@ -59,20 +61,21 @@ __device__ __forceinline__ void kept_issue(const MlpDims& d, const float* __rest
   const float* fbase = feats + row0 * d.F;
   asm volatile("" : "+s"(fbase));
   const auto rf = buffer_rsrc(fbase, nv * d.F * 4);
-  const auto ra = buffer_rsrc(act + row0 * 256, nv * 1024);
+  const auto ra = buffer_rsrc(reinterpret_cast<const char*>(act) + row0 * kKeptRowBytes<BF>,
+                              nv * kKeptRowBytes<BF>);
   const auto rs = buffer_rsrc(rgb + row0 * 3, nv * 12);
   const auto rg = buffer_rsrc(grgb + row0 * 3, nv * 12);
   const auto rd = buffer_rsrc(dirs + ray0 * 3, nray * 12);
   const int fp4 = d.FP / 4, f4 = d.F / 4, n4 = TM * fp4;
   const int t = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < KeptRegs<TM, NT>::kF; ++k) {
+  for (int k = 0; k < KeptRegs<TM, NT, BF>::kF; ++k) {
     const int e = t + k * NT, i = e / fp4, c4 = e - i * fp4;
     const int off = (e < n4 && c4 < f4) ? (i * d.F + 4 * c4) * 4 : kOOB;
     r.vf[k] = __builtin_amdgcn_raw_buffer_load_b128(rf, off, 0, 0);
   }
 #pragma unroll
-  for (int k = 0; k < KeptRegs<TM, NT>::kA; ++k)
+  for (int k = 0; k < KeptRegs<TM, NT, BF>::kA; ++k)
     r.va[k] = __builtin_amdgcn_raw_buffer_load_b128(ra, (t + k * NT) * 16, 0, 0);
   const int i3 = t / 3, c3 = t - 3 * i3;
   const bool own3 = t < TM * 3;
@@ -88,14 +91,14 @@ __device__ __forceinline__ void kept_issue(const MlpDims& d, const float* __rest
 
 // Stores the issued tile: features (Fs, zero padded to FP), directions (Ds),
 // relu(z1) (H1), the layer-3 input (H2) and dZ3 (Z3, zero past the colours).
-template <int TM, int NT>
-__device__ __forceinline__ void kept_commit(const MlpDims& d, const KeptRegs<TM, NT>& r,
+template <int TM, int NT, bool BF>
+__device__ __forceinline__ void kept_commit(const MlpDims& d, const KeptRegs<TM, NT, BF>& r,
                                             float* Fs, float* Ds, float* H1, float* H2,
                                             float* Z3) {
   const int fp4 = d.FP / 4, n4 = TM * fp4;
   const int t = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < KeptRegs<TM, NT>::kF; ++k) {
+  for (int k = 0; k < KeptRegs<TM, NT, BF>::kF; ++k) {
     const int e = t + k * NT, i = e / fp4, c4 = e - i * fp4;
     if (e < n4)
       *reinterpret_cast<float4*>(Fs + i * d.ldF + 4 * c4) = __builtin_bit_cast(float4, r.vf[k]);
@@ -106,65 +109,23 @@ __device__ __forceinline__ void kept_commit(const MlpDims& d, const KeptRegs<TM,
     Z3[i3 * d.ld3 + c3] = r.g * r.s * (1.0f - r.s);
   }
 #pragma unroll
-  for (int k = 0; k < KeptRegs<TM, NT>::kA; ++k) {
-    const int e = t + k * NT, i = e >> 6, c4 = e & 63;
-    float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
-    *reinterpret_cast<float4*>(dst) = __builtin_bit_cast(float4, r.va[k]);
+  for (int k = 0; k < KeptRegs<TM, NT, BF>::kA; ++k) {
+    if constexpr (BF) {
+      const int e = t + k * NT, i = e >> 5, c8 = e & 31;
+      float* dst = c8 < 16 ? H1 + i * d.ldU + 8 * c8 : H2 + i * d.ldU + 8 * (c8 - 16);
+      float4 lo, hi;
+      bf16x8_to_f32(__builtin_bit_cast(int4, r.va[k]), lo, hi);
+      *reinterpret_cast<float4*>(dst) = lo;
+      *reinterpret_cast<float4*>(dst + 4) = hi;
+    } else {
+      const int e = t + k * NT, i = e >> 6, c4 = e & 63;
+      float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
+      *reinterpret_cast<float4*>(dst) = __builtin_bit_cast(float4, r.va[k]);
+    }
   }
   for (int e = t; e < TM * (d.Z3W - 3); e += NT) {
     const int i = e / (d.Z3W - 3), c = 3 + e % (d.Z3W - 3);
     Z3[i * d.ld3 + c] = 0.0f;
-  }
-}
-
-// Encoding of the tile (networks.py:13-35, 68-76) from the two layer-0 parts:
-// X[:, 0:S] = h0 = XA + XB (XB stride ldB), then X[:, S:EP] =
-// [d, FE(h0, nf), FE(d, nv), 0...] with FE(v, n)[j*2n + q] = sin(2^q v_j) and
-// [j*2n + n + q] = cos(2^q v_j). q = 0 by sincosf; q = 1 from it by
-// sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a); q >= 2
-// by sincosf again.
-template <int TM>
-__device__ __forceinline__ void encode_two_part(const MlpDims& d, float* X, const float* XB,
-                                                int ldB, const float* Ds) {
-  const int fe0 = d.S + 3;
-  const int fe1 = fe0 + 2 * d.nf * d.S;
-  const int fe2 = fe1 + 2 * d.nv * 3;
-  const int nplain = 3 + (d.EP - fe2);
-  for (int e = threadIdx.x; e < TM * nplain; e += blockDim.x) {
-    const int i = e / nplain, t = e % nplain;
-    const int c = t < 3 ? d.S + t : fe2 + (t - 3);
-    X[i * d.ldE + c] = t < 3 ? Ds[i * 3 + t] : 0.0f;
-  }
-  const int nval = d.S + 3;  // values per row: h0, then the direction
-  for (int e = threadIdx.x; e < TM * nval; e += blockDim.x) {
-    const int i = e / nval, u = e % nval;
-    const bool feat = u < d.S;
-    const int j = feat ? u : u - d.S;
-    const int n = feat ? d.nf : d.nv;
-    float v;
-    if (feat) {
-      v = X[i * d.ldE + j] + XB[i * ldB + j];
-      X[i * d.ldE + j] = v;
-    } else {
-      v = Ds[i * 3 + j];
-    }
-    float* fe = X + i * d.ldE + (feat ? fe0 : fe1) + j * 2 * n;
-    float sv, cv;
-    sincosf(v, &sv, &cv);
-    if (n > 0) {
-      fe[0] = sv;
-      fe[n] = cv;
-    }
-    if (n > 1) {
-      fe[1] = 2.0f * sv * cv;
-      fe[n + 1] = (cv - sv) * (cv + sv);
-    }
-    for (int q = 2; q < n; ++q) {
-      float s2, c2;
-      sincosf(v * (float)(1 << q), &s2, &c2);
-      fe[q] = s2;
-      fe[n + q] = c2;
-    }
   }
 }
 
@@ -224,12 +185,12 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
                 "layer-0 K parts in whole MFMA steps");
 
   const int64_t ntiles = (M + TM - 1) / TM;
-  KeptRegs<TM, NT> nx;
-  kept_issue<TM, NT>(d, feats, dirs, act, rgb_in, grgb, (int64_t)blockIdx.x * TM, M, rpr, nx);
+  KeptRegs<TM, NT, BF> nx;
+  kept_issue<TM, NT, BF>(d, feats, dirs, act, rgb_in, grgb, (int64_t)blockIdx.x * TM, M, rpr, nx);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t row0 = tile * TM;
     __syncthreads();
-    kept_commit<TM, NT>(d, nx, Fs, Ds, H1, H2, Z3);
+    kept_commit<TM, NT, BF>(d, nx, Fs, Ds, H1, H2, Z3);
     __syncthreads();
     // T1: layer 0, one (column tile, row tile, K part) per wave -> X / H0 ;
     // dW3 += H2^T dZ3 ; db3
@@ -259,7 +220,7 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
     if (threadIdx.x < 3 * TM) db3 += Z3[(threadIdx.x / 3) * d.ld3 + threadIdx.x % 3];
     __syncthreads();
     // T2: encoding ; dZ2 = (dZ3 @ W3^T) * (h2 > 0) over H2 ; db2
-    encode_two_part<TM>(d, X, H0, d.ldS, Ds);
+    encode_tile<TM, true>(d, X, Ds, H0, d.ldS);
     {
       const int ct = wave;  // 8 column tiles, all row tiles per item
       f32x4 acc[RT];
@@ -279,8 +240,8 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
     __syncthreads();
     // T3: the next tile's loads (nothing in this phase waits on vmcnt) ;
     // dW2 += H1^T dZ2
-    kept_issue<TM, NT>(d, feats, dirs, act, rgb_in, grgb, row0 + (int64_t)gridDim.x * TM, M, rpr,
-                       nx);
+    kept_issue<TM, NT, BF>(d, feats, dirs, act, rgb_in, grgb, row0 + (int64_t)gridDim.x * TM, M,
+                           rpr, nx);
     wgrad_phase<TM, NW, BF, kSlots, kSlot2, kCnt2>(wacc, wave, wt.n2, kU / 16, H1, d.ldU, H2,
                                                    d.ldU, part, gp.w2, kU, kU);
     __syncthreads();

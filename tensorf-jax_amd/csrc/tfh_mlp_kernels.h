@@ -60,6 +60,36 @@ struct MlpDims {
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// Kept activations (the forward's relu(z1) and layer-3 input, 256 values a
+// row, for the backward): FP32 rows, or BF16 rows with the BF16 MLP, whose
+// backward only reads them as BF16 MFMA operands and as ReLU signs (the
+// rounding keeps signs and zeros), so BF16 storage changes no result and
+// halves the traffic.
+template <bool BF>
+constexpr int kKeptRowBytes = 256 * (BF ? 2 : 4);
+
+__device__ __forceinline__ void bf16x8_to_f32(const int4& v, float4& lo, float4& hi) {
+  // a BF16 value is the upper half of its FP32 bit pattern
+  const unsigned w[4] = {(unsigned)v.x, (unsigned)v.y, (unsigned)v.z, (unsigned)v.w};
+  lo = make_float4(__uint_as_float(w[0] << 16), __uint_as_float(w[0] & 0xffff0000u),
+                   __uint_as_float(w[1] << 16), __uint_as_float(w[1] & 0xffff0000u));
+  hi = make_float4(__uint_as_float(w[2] << 16), __uint_as_float(w[2] & 0xffff0000u),
+                   __uint_as_float(w[3] << 16), __uint_as_float(w[3] & 0xffff0000u));
+}
+
+// Stores four kept values as FP32 or BF16 (round to nearest even).
+template <bool BF>
+__device__ __forceinline__ void store_kept4(void* row, int col, const float4& v) {
+  if constexpr (BF) {
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    bf16x4 b;
+    b[0] = (__bf16)v.x; b[1] = (__bf16)v.y; b[2] = (__bf16)v.z; b[3] = (__bf16)v.w;
+    *reinterpret_cast<bf16x4*>(static_cast<__bf16*>(row) + col) = b;
+  } else {
+    *reinterpret_cast<float4*>(static_cast<float*>(row) + col) = v;
+  }
+}
+
 // FP32 pads K to 16, BF16 to 32; row strides are 4 mod 8 floats: 16-byte
 // aligned rows whose 128-bit reads by 8 consecutive rows hit 8 distinct bank
 // groups.
@@ -215,7 +245,7 @@ __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
 // issued before the first LDS store and the tile pays one memory latency.
 // With CAMS also each row's camera index (Cs, clamped to [0, ncam), from the
 // per-ray cams). Requires F % 4 == 0 and 0 < rpr < 2^31.
-template <int TM, int NT, bool ACT, bool CAMS = false>
+template <int TM, int NT, bool ACT, bool CAMS = false, bool ABF = false>
 __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats,
                                const float* __restrict__ dirs, const float* __restrict__ act,
                                const float* __restrict__ rgb, const float* __restrict__ grgb,
@@ -223,8 +253,9 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
                                float* H1, float* H2, float* Z3,
                                const uint32_t* __restrict__ cams = nullptr, int* Cs = nullptr) {
   typedef int i32x4 __attribute__((ext_vector_type(4)));
-  constexpr int kA4 = ACT ? TM * 64 / NT : 1;  // float4 of activations per thread
-  static_assert(TM * 64 % NT == 0 && TM * 3 <= NT, "tile shape");
+  // 16-byte loads of kept activations per thread (4 FP32 or 8 BF16 values)
+  constexpr int kA4 = ACT ? TM * (ABF ? 32 : 64) / NT : 1;
+  static_assert(TM * (ABF ? 32 : 64) % NT == 0 && TM * 3 <= NT, "tile shape");
   const int nv = (int)min((int64_t)TM, M - row0);
   const int64_t ray0 = row0 / rpr;
   const unsigned rem0 = (unsigned)(row0 - ray0 * rpr);
@@ -235,7 +266,11 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
   const float* fbase = feats + row0 * d.F;
   asm volatile("" : "+s"(fbase));
   const auto rf = buffer_rsrc(fbase, nv * d.F * 4);
-  const auto ra = buffer_rsrc(ACT ? act + row0 * 256 : feats, ACT ? nv * 1024 : 0);
+  const auto ra = buffer_rsrc(
+      ACT ? static_cast<const void*>(reinterpret_cast<const char*>(act) +
+                                     row0 * kKeptRowBytes<ABF>)
+          : feats,
+      ACT ? nv * kKeptRowBytes<ABF> : 0);
   const auto rs = buffer_rsrc(ACT ? rgb + row0 * 3 : feats, ACT ? nv * 12 : 0);
   const auto rg = buffer_rsrc(ACT ? grgb + row0 * 3 : feats, ACT ? nv * 12 : 0);
   const auto rd = buffer_rsrc(dirs + ray0 * 3, nray * 12);
@@ -290,9 +325,18 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
   if constexpr (ACT) {
 #pragma unroll
     for (int k = 0; k < kA4; ++k) {
-      const int e = t + k * NT, i = e >> 6, c4 = e & 63;
-      float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
-      *reinterpret_cast<float4*>(dst) = __builtin_bit_cast(float4, va[k]);
+      if constexpr (ABF) {
+        const int e = t + k * NT, i = e >> 5, c8 = e & 31;
+        float* dst = c8 < 16 ? H1 + i * d.ldU + 8 * c8 : H2 + i * d.ldU + 8 * (c8 - 16);
+        float4 lo, hi;
+        bf16x8_to_f32(__builtin_bit_cast(int4, va[k]), lo, hi);
+        *reinterpret_cast<float4*>(dst) = lo;
+        *reinterpret_cast<float4*>(dst + 4) = hi;
+      } else {
+        const int e = t + k * NT, i = e >> 6, c4 = e & 63;
+        float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
+        *reinterpret_cast<float4*>(dst) = __builtin_bit_cast(float4, va[k]);
+      }
     }
     if (own3) Z3[i3 * d.ld3 + c3] = g * s * (1.0f - s);
     for (int e = t; e < TM * (d.Z3W - 3); e += NT) {
@@ -303,11 +347,21 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
 }
 
 // Fills X[:, S:EP] = [d, FE(h0, nf), FE(d, nv), 0...] from h0 = X[:, 0:S]
-// (networks.py:13-35, 68-76): FE(v, n)[j*2n + q] = sin(2^q v_j) for q < n and
-// sin(2^(q-n) v_j + pi/2) = cos(2^(q-n) v_j) otherwise; one sincosf per
-// (row, value, frequency) writes both.
-template <int TM>
-__device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
+// (networks.py:13-35, 68-76): FE(v, n)[j*2n + q] = sin(2^q v_j) and
+// [j*2n + n + q] = cos(2^q v_j).
+// The forward (DOUBLE = false) takes one sincosf per (row, value, frequency).
+// The backward's recompute (DOUBLE = true) takes one per (row, value): q = 0
+// by sincosf, q = 1 from it by sin 2a = 2 sin a cos a and
+// cos 2a = (cos a - sin a)(cos a + sin a), q >= 2 by sincosf again; its
+// values feed only the weight gradient of layer 1 and the encoding's own
+// backward, where the identities' ~2 ulp stay below the gradient tolerance
+// (taken in the forward, their bias shifted the outputs a whole-batch
+// gradient sums over a million rows). With XB (stride ldB), h0 is first
+// formed as X[:, 0:S] + XB[:, 0:S] and written back to X (a layer 0
+// computed in two K parts).
+template <int TM, bool DOUBLE = false>
+__device__ void encode_tile(const MlpDims& d, float* X, const float* Ds,
+                            const float* XB = nullptr, int ldB = 0) {
   const int fe0 = d.S + 3;
   const int fe1 = fe0 + 2 * d.nf * d.S;
   const int fe2 = fe1 + 2 * d.nv * 3;
@@ -318,19 +372,55 @@ __device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
     const int c = t < 3 ? d.S + t : fe2 + (t - 3);
     X[i * d.ldE + c] = t < 3 ? Ds[i * 3 + t] : 0.0f;
   }
-  const int nfe = d.S * d.nf + 3 * d.nv;  // (value, frequency) pairs per row
-  for (int e = threadIdx.x; e < TM * nfe; e += blockDim.x) {
-    const int i = e / nfe, t = e % nfe;
-    const bool feat = t < d.S * d.nf;
-    const int u = feat ? t : t - d.S * d.nf;
+  if constexpr (!DOUBLE) {
+    const int nfe = d.S * d.nf + 3 * d.nv;  // (value, frequency) pairs per row
+    for (int e = threadIdx.x; e < TM * nfe; e += blockDim.x) {
+      const int i = e / nfe, t = e % nfe;
+      const bool feat = t < d.S * d.nf;
+      const int u = feat ? t : t - d.S * d.nf;
+      const int n = feat ? d.nf : d.nv;
+      const int j = u / n, q = u % n;
+      const float src = feat ? X[i * d.ldE + j] : Ds[i * 3 + j];
+      float sv, cv;
+      sincosf(src * (float)(1 << q), &sv, &cv);
+      const int c = (feat ? fe0 : fe1) + j * 2 * n + q;
+      X[i * d.ldE + c] = sv;
+      X[i * d.ldE + c + n] = cv;
+    }
+    return;
+  }
+  const int nval = d.S + 3;  // values per row: h0, then the direction
+  for (int e = threadIdx.x; e < TM * nval; e += blockDim.x) {
+    const int i = e / nval, u = e % nval;
+    const bool feat = u < d.S;
+    const int j = feat ? u : u - d.S;
     const int n = feat ? d.nf : d.nv;
-    const int j = u / n, q = u % n;
-    const float src = feat ? X[i * d.ldE + j] : Ds[i * 3 + j];
+    float v;
+    if (feat) {
+      v = X[i * d.ldE + j];
+      if (XB) {
+        v += XB[i * ldB + j];
+        X[i * d.ldE + j] = v;
+      }
+    } else {
+      v = Ds[i * 3 + j];
+    }
+    if (n == 0) continue;
+    float* fe = X + i * d.ldE + (feat ? fe0 : fe1) + j * 2 * n;
     float sv, cv;
-    sincosf(src * (float)(1 << q), &sv, &cv);
-    const int c = (feat ? fe0 : fe1) + j * 2 * n + q;
-    X[i * d.ldE + c] = sv;
-    X[i * d.ldE + c + n] = cv;
+    sincosf(v, &sv, &cv);
+    fe[0] = sv;
+    fe[n] = cv;
+    if (n > 1) {
+      fe[1] = 2.0f * sv * cv;
+      fe[n + 1] = (cv - sv) * (cv + sv);
+    }
+    for (int q = 2; q < n; ++q) {
+      float s2, c2;
+      sincosf(v * (float)(1 << q), &s2, &c2);
+      fe[q] = s2;
+      fe[n + q] = c2;
+    }
   }
 }
 
@@ -342,12 +432,13 @@ __device__ void encode_tile(const MlpDims& d, float* X, const float* Ds) {
 // spreads a layer over more waves.
 // L12 == false stops after the encoding (H1 and H2 come from elsewhere).
 template <int TM, int NW, bool BF, int RTI, bool L12 = true>
-// h2g (optional, rows < nvalid): relu(z2) also stored to global memory, row
-// stride 256 floats (the forward's kept activations with an embedding).
+// h2g (optional, rows < nvalid): relu(z2) also stored to global memory, rows
+// of 256 kept values (FP32, or BF16 with the BF16 MLP: the forward's kept
+// activations with an embedding).
 __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
                                const float* Fs, float* X, float* H0, float* H1, float* H2p,
                                float* H2, const float* Ds, const int* Cs,
-                               float* __restrict__ h2g = nullptr, int nvalid = 0) {
+                               void* __restrict__ h2g = nullptr, int nvalid = 0) {
   constexpr int RG = TM / 16 / RTI;  // row groups
   // B prefetch depth (K per pass): the whole strip with few waves, 64 when the
   // workgroup's waves share the register file (16 waves)
@@ -411,7 +502,12 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
           out = emb[col - kU / 2] * h + emb[col];
         }
         if (H2p != H2) H2p[row * d.ldU + col] = h;
-        if (h2g && row < nvalid) h2g[row * 256 + col] = h;
+        if (h2g && row < nvalid) {
+          if constexpr (BF)
+            static_cast<__bf16*>(h2g)[row * 256 + col] = (__bf16)h;
+          else
+            static_cast<float*>(h2g)[row * 256 + col] = h;
+        }
         H2[row * d.ldU + col] = out;
       }
   }
@@ -487,16 +583,17 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   // (the layer-3 input without one); with an embedding layer 2 stores the
   // second half itself
   const bool film_keep = act && d.ncam > 0;
+  char* kept = reinterpret_cast<char*>(act);
+  constexpr int kRow = kKeptRowBytes<BF>;
   forward_hidden<TM, NW, BF, TM / 16>(d, w, op, regA, regB, nullptr, regA, regB, regB, Ds, Cs,
-                                      film_keep ? act + row0 * 256 + 128 : nullptr,
+                                      film_keep ? kept + row0 * kRow + kRow / 2 : nullptr,
                                       (int)min((int64_t)TM, M - row0));
   if (act) {
     for (int e = threadIdx.x; e < TM * 64; e += blockDim.x) {
       const int i = e >> 6, c4 = e & 63;
       if (row0 + i < M && (c4 < 32 || !film_keep)) {
         const float* src = c4 < 32 ? regA + i * d.ldU + 4 * c4 : regB + i * d.ldU + 4 * (c4 - 32);
-        *reinterpret_cast<float4*>(act + (row0 + i) * 256 + 4 * c4) =
-            *reinterpret_cast<const float4*>(src);
+        store_kept4<BF>(kept + (row0 + i) * kRow, 4 * c4, *reinterpret_cast<const float4*>(src));
       }
     }
   }
@@ -811,7 +908,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     __syncthreads();
     if constexpr (ACT) {
       // with an embedding the kept second half is relu(z2) before it (H2p)
-      load_tile_rows<TM, NW * 64, true, FILM>(d, feats, dirs, act, rgb_in, grgb, row0, M, rpr, Fs,
+      load_tile_rows<TM, NW * 64, true, FILM, BF>(d, feats, dirs, act, rgb_in, grgb, row0, M, rpr, Fs,
                                               Ds, H1, FILM ? H2p : H2, Z3, cams, Cs);
     } else {
       load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
