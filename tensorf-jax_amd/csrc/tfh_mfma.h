@@ -5,6 +5,8 @@
 // BF16 (K=32): A[l%16][k0 + 8(l/16) + e], B[k0 + 8(l/16) + e][l%16], e < 8
 #pragma once
 
+#include <type_traits>
+
 #include "tfh_common.h"
 
 namespace tfh {
@@ -134,9 +136,19 @@ struct PackedB32 {
 // four B operands one 16-byte buffer load. The sum over k is unchanged; only
 // the order of the FP32 accumulation differs from a k-ascending chain.
 // Requires K % 16 == 0, lda % 4 == 0 and a 16-byte aligned A.
-template <int NR, int KMAX, class B4>
+// The B chunks an FP32 strip requests first (the ring's PD), loaded ahead of
+// the strip: before the barrier that precedes its phase, so that their L2
+// round trip overlaps the barrier wait instead of following it.
+template <int KMAX>
+struct PreF32 {
+  static constexpr int NS = KMAX / 16, PD = NS < 4 ? NS : 4;
+  float4 bb[PD];
+};
+
+template <int NR, int KMAX, class B4, bool PRE = false>
 __device__ __forceinline__ void mma_strip_f32(const float* __restrict__ A, int lda, int K, B4 b,
-                                              int col0, f32x4 (&acc)[NR]) {
+                                              int col0, f32x4 (&acc)[NR],
+                                              const PreF32<KMAX>* pre = nullptr) {
   const int lane = lane_here();
   const int r = lane & 15, kq = lane >> 4;
   constexpr int NS = KMAX / 16;
@@ -150,7 +162,12 @@ __device__ __forceinline__ void mma_strip_f32(const float* __restrict__ A, int l
     float4 bb[NS];
 #pragma unroll
     for (int i = 0; i < PD; ++i)
-      if (kc + 16 * i < K) bb[i] = b.load4(kc + 16 * i + 4 * kq, col0 + r);
+      if (kc + 16 * i < K) {
+        if (PRE && kc == 0)
+          bb[i] = pre->bb[i];
+        else
+          bb[i] = b.load4(kc + 16 * i + 4 * kq, col0 + r);
+      }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
@@ -176,6 +193,21 @@ template <int NR, int KMAX>
 __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB32 b,
                                           int col0, f32x4 (&acc)[NR]) {
   mma_strip_f32<NR, KMAX>(A, lda, K, b, col0, acc);
+}
+
+template <int KMAX>
+__device__ __forceinline__ void strip_preload(PackedB32 b, int K, int col0, PreF32<KMAX>& p) {
+  const int lane = lane_here();
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < PreF32<KMAX>::PD; ++i)
+    if (16 * i < K) p.bb[i] = b.load4(16 * i + 4 * kq, col0 + r);
+}
+
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB32 b,
+                                          int col0, f32x4 (&acc)[NR], const PreF32<KMAX>& p) {
+  mma_strip_f32<NR, KMAX, PackedB32, true>(A, lda, K, b, col0, acc, &p);
 }
 
 template <int NR, int KMAX>
@@ -217,9 +249,18 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
   }
 }
 
-template <int NR, int KMAX, class B16>
+// The B chunks of a BF16 strip (all of them: the strip loads its whole
+// column before the first MFMA), loaded ahead of the strip.
+template <int KMAX>
+struct PreB16 {
+  static constexpr int NS = (KMAX + 31) / 32;
+  bf16x8 bb[NS];
+};
+
+template <int NR, int KMAX, class B16, bool PRE = false>
 __device__ __forceinline__ void mma_strip_b16(const float* __restrict__ A, int lda, int K, B16 b,
-                                              int col0, f32x4 (&acc)[NR]) {
+                                              int col0, f32x4 (&acc)[NR],
+                                              const PreB16<KMAX>* pre = nullptr) {
   const int lane = lane_here();
   const int r = lane & 15, kq = lane >> 4;
   constexpr int NS = (KMAX + 31) / 32;
@@ -227,7 +268,12 @@ __device__ __forceinline__ void mma_strip_b16(const float* __restrict__ A, int l
     bf16x8 bb[NS];
 #pragma unroll
     for (int i = 0; i < NS; ++i)
-      if (kc + 32 * i < K) bb[i] = b.load8(kc + 32 * i + 8 * kq, col0 + r);
+      if (kc + 32 * i < K) {
+        if (PRE && kc == 0)
+          bb[i] = pre->bb[i];
+        else
+          bb[i] = b.load8(kc + 32 * i + 8 * kq, col0 + r);
+      }
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
       if (kc + 32 * i < K) {
@@ -248,6 +294,24 @@ __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, 
                                           int col0, f32x4 (&acc)[NR]) {
   mma_strip_b16<NR, KMAX>(A, lda, K, b, col0, acc);
 }
+
+template <int KMAX>
+__device__ __forceinline__ void strip_preload(PackedB b, int K, int col0, PreB16<KMAX>& p) {
+  const int lane = lane_here();
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < PreB16<KMAX>::NS; ++i)
+    if (32 * i < K) p.bb[i] = b.load8(32 * i + 8 * kq, col0 + r);
+}
+
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB b,
+                                          int col0, f32x4 (&acc)[NR], const PreB16<KMAX>& p) {
+  mma_strip_b16<NR, KMAX, PackedB, true>(A, lda, K, b, col0, acc, &p);
+}
+
+template <bool BF, int KMAX>
+using StripPre = typename std::conditional<BF, PreB16<KMAX>, PreF32<KMAX>>::type;
 
 template <int NR, int KMAX>
 __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, LdsB b,

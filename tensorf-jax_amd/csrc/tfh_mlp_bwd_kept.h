@@ -244,13 +244,16 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
                            rpr, nx);
     wgrad_phase<TM, NW, BF, kSlots, kSlot2, kCnt2>(wacc, wave, wt.n2, kU / 16, H1, d.ldU, H2,
                                                    d.ldU, part, gp.w2, kU, kU);
+    // the dZ1 strip's first B chunks, requested before the barrier
+    StripPre<BF, kU> pre2;
+    strip_preload<kU>(op.g2(), kU, 16 * wave, pre2);
     __syncthreads();
     // T4: dZ1 = (dZ2 @ W2^T) * (h1 > 0) over H1 ; db1
     {
       const int ct = wave;
       f32x4 acc[RT];
       zero_acc(acc);
-      mma_strip<RT, kU>(H2, d.ldU, kU, op.g2(), 16 * ct, acc);
+      mma_strip<RT, kU>(H2, d.ldU, kU, op.g2(), 16 * ct, acc, pre2);
       const int col = 16 * ct + (lane & 15);
 #pragma unroll
       for (int q = 0; q < 4 * RT; ++q) {
@@ -314,6 +317,9 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
         H0[i * d.ldS + j] = g;
       }
     }
+    // the first dF item's B chunks (item wave: column tile wave / RT)
+    StripPre<BF, kSP> pre0;
+    strip_preload<kSP>(op.g0(), kSP, 16 * (wave / RT), pre0);
     __syncthreads();
     // T7: dW0 += Fs^T dH0 ; dF = dH0 @ W0^T -> gfeat
     wgrad_phase<TM, NW, BF, kSlots, kLSlot0, kCnt0, true>(wacc, wave, wt.n0, 2, Fs, d.ldF, H0,
@@ -322,7 +328,10 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
       const int ct = it / RT, r0 = 16 * (it % RT);
       f32x4 acc[1];
       zero_acc(acc);
-      mma_strip<1, kSP>(H0 + r0 * d.ldS, d.ldS, kSP, op.g0(), 16 * ct, acc);
+      if (it == wave)
+        mma_strip<1, kSP>(H0 + r0 * d.ldS, d.ldS, kSP, op.g0(), 16 * ct, acc, pre0);
+      else
+        mma_strip<1, kSP>(H0 + r0 * d.ldS, d.ldS, kSP, op.g0(), 16 * ct, acc);
       const int col = 16 * ct + (lane & 15);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
