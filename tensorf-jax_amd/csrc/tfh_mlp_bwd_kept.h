@@ -191,6 +191,19 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
     const int64_t row0 = tile * TM;
     __syncthreads();
     kept_commit<TM, NT, BF>(d, nx, Fs, Ds, H1, H2, Z3);
+    // the layer-0 item's B chunks (item wave: K part wave & 1, column tile
+    // (wave >> 1) & 1), requested before the barrier
+    constexpr int K0A = BF ? 96 : 80;
+    StripPre<BF, K0A> pre0l;
+    {
+      const int kp = wave & 1, ct = (wave >> 1) & 1;
+      if constexpr (BF)
+        strip_preload<K0A>(PackedB(pk.f0 + (kp ? K0A : 0), d.FP, 32), kp ? d.FP - K0A : K0A,
+                           16 * ct, pre0l);
+      else
+        strip_preload<K0A>(PackedB32(pk.ff0 + (kp ? K0A : 0), d.FP, 32),
+                           kp ? d.FP - K0A : K0A, 16 * ct, pre0l);
+    }
     __syncthreads();
     // T1: layer 0, one (column tile, row tile, K part) per wave -> X / H0 ;
     // dW3 += H2^T dZ3 ; db3
@@ -198,14 +211,23 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
       const int kp = it & 1, ct = (it >> 1) & 1, r0 = 16 * (it >> 2);
       f32x4 acc[1];
       zero_acc(acc);
+      static_assert(K0A == KA, "preloaded layer-0 chunks match the K split");
       if constexpr (BF) {
         const PackedB b(pk.f0 + (kp ? KA : 0), d.FP, 32);
-        mma_strip<1, 96>(Fs + r0 * d.ldF + (kp ? KA : 0), d.ldF, kp ? d.FP - KA : KA, b,
-                         16 * ct, acc);
+        if (it == wave)
+          mma_strip<1, KA>(Fs + r0 * d.ldF + (kp ? KA : 0), d.ldF, kp ? d.FP - KA : KA, b,
+                           16 * ct, acc, pre0l);
+        else
+          mma_strip<1, KA>(Fs + r0 * d.ldF + (kp ? KA : 0), d.ldF, kp ? d.FP - KA : KA, b,
+                           16 * ct, acc);
       } else {
         const PackedB32 b(pk.ff0 + (kp ? KA : 0), d.FP, 32);
-        mma_strip<1, 80>(Fs + r0 * d.ldF + (kp ? KA : 0), d.ldF, kp ? d.FP - KA : KA, b,
-                         16 * ct, acc);
+        if (it == wave)
+          mma_strip<1, KA>(Fs + r0 * d.ldF + (kp ? KA : 0), d.ldF, kp ? d.FP - KA : KA, b,
+                           16 * ct, acc, pre0l);
+        else
+          mma_strip<1, KA>(Fs + r0 * d.ldF + (kp ? KA : 0), d.ldF, kp ? d.FP - KA : KA, b,
+                           16 * ct, acc);
       }
       float* dst = kp ? H0 : X;
       const int ld = kp ? d.ldS : d.ldE;
@@ -218,6 +240,9 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
     wgrad_phase<TM, NW, BF, kSlots, kLSlot3, kCnt3, true>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3,
                                                           d.ld3, part, gp.w3, kU, 3, Wl);
     if (threadIdx.x < 3 * TM) db3 += Z3[(threadIdx.x / 3) * d.ld3 + threadIdx.x % 3];
+    // the dZ2 strip's B chunks (column tile wave), requested before the barrier
+    StripPre<BF, 32> pre3;
+    strip_preload<32>(op.g3(), op.k3(), 16 * wave, pre3);
     __syncthreads();
     // T2: encoding ; dZ2 = (dZ3 @ W3^T) * (h2 > 0) over H2 ; db2
     encode_tile<TM, true>(d, X, Ds, H0, d.ldS);
@@ -225,7 +250,7 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
       const int ct = wave;  // 8 column tiles, all row tiles per item
       f32x4 acc[RT];
       zero_acc(acc);
-      mma_strip<RT, 32>(Z3, d.ld3, op.k3(), op.g3(), 16 * ct, acc);
+      mma_strip<RT, 32>(Z3, d.ld3, op.k3(), op.g3(), 16 * ct, acc, pre3);
       const int col = 16 * ct + (lane & 15);
 #pragma unroll
       for (int q = 0; q < 4 * RT; ++q) {
