@@ -54,6 +54,28 @@ __device__ __forceinline__ void tap_values(const float* __restrict__ plane,
   pv = ((a.w00 * t00 + a.w01 * t01) + a.w10 * t10) + a.w11 * t11;
 }
 
+// line and plane values of channels c4 .. c4+3 for one mode: one 16-byte
+// load per tap (C and c4 multiples of 4); each channel's arithmetic is that
+// of tap_values.
+__device__ __forceinline__ void tap_values4(const float* __restrict__ plane,
+                                            const float* __restrict__ line, int C, int c4,
+                                            const TapAddr& a, float4& lv, float4& pv) {
+  const float4 l0 = *reinterpret_cast<const float4*>(line + (size_t)a.line0 * C + c4);
+  const float4 l1 = *reinterpret_cast<const float4*>(line + (size_t)a.line1 * C + c4);
+  const float4 t00 = *reinterpret_cast<const float4*>(plane + (size_t)a.p00 * C + c4);
+  const float4 t01 = *reinterpret_cast<const float4*>(plane + (size_t)a.p01 * C + c4);
+  const float4 t10 = *reinterpret_cast<const float4*>(plane + (size_t)a.p10 * C + c4);
+  const float4 t11 = *reinterpret_cast<const float4*>(plane + (size_t)a.p11 * C + c4);
+  lv.x = a.wl0 * l0.x + a.wl1 * l1.x;
+  lv.y = a.wl0 * l0.y + a.wl1 * l1.y;
+  lv.z = a.wl0 * l0.z + a.wl1 * l1.z;
+  lv.w = a.wl0 * l0.w + a.wl1 * l1.w;
+  pv.x = ((a.w00 * t00.x + a.w01 * t01.x) + a.w10 * t10.x) + a.w11 * t11.x;
+  pv.y = ((a.w00 * t00.y + a.w01 * t01.y) + a.w10 * t10.y) + a.w11 * t11.y;
+  pv.z = ((a.w00 * t00.z + a.w01 * t01.z) + a.w10 * t10.z) + a.w11 * t11.z;
+  pv.w = ((a.w00 * t00.w + a.w01 * t01.w) + a.w10 * t10.w) + a.w11 * t11.w;
+}
+
 // Adds `val` at dst[key*C + c] once per distinct `key` among the 64/LPS
 // groups of LPS lanes of the wave (all lanes of the wave must call it
 // together): each group sums the values of the groups with its key in the

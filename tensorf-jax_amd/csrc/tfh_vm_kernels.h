@@ -103,9 +103,54 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_gather(tfh_vm vm, Src src
   }
 }
 
+// Row-major gather with 16-byte taps: LPS lanes per sample, lane l owns
+// channels 4l .. 4l+3 of every mode (LPS = C/4 rounded up to a power of
+// two); a quarter of the load instructions of k_vm_gather, the same values.
+template <class Src, int LPS>
+__global__ void __launch_bounds__(kGatherThreads) k_vm_gather4(tfh_vm vm, Src src, int64_t M,
+                                                               float* __restrict__ out) {
+  constexpr int SPB = kGatherThreads / LPS;
+  const int grp = threadIdx.x / LPS, l = threadIdx.x % LPS;
+  const int64_t s = (int64_t)blockIdx.x * SPB + grp;
+  const int G = (int)vm.grid_dim, C = (int)vm.channels;
+  const int c4 = 4 * l;
+  if (s >= M || c4 >= C) return;  // no barrier or lane exchange below
+  float x = 0.f, y = 0.f, z = 0.f;
+  src.get(s, x, y, z);
+  ModeTaps mt[3];
+  mode_taps(x, y, z, G, mt);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    TapAddr a;
+    tap_addrs(mt[m], m, G, a);
+    float4 lv, pv;
+    tap_values4(vm.plane, vm.line, C, c4, a, lv, pv);
+    // tensor_vm.py:167
+    *reinterpret_cast<float4*>(out + s * 3 * C + m * C + c4) =
+        make_float4(lv.x * pv.x, lv.y * pv.y, lv.z * pv.z, lv.w * pv.w);
+  }
+}
+
 template <class Src, bool ROW_MAJOR>
 inline int32_t launch_vm_gather(const tfh_vm& vm, const Src& src, int64_t M, float* out,
                                 hipStream_t st) {
+  const int C = (int)vm.channels;
+  if (ROW_MAJOR && C % 4 == 0 && C <= 64 && (reinterpret_cast<uintptr_t>(out) & 15) == 0 &&
+      (reinterpret_cast<uintptr_t>(vm.plane) & 15) == 0 &&
+      (reinterpret_cast<uintptr_t>(vm.line) & 15) == 0) {
+    const int lps = C <= 8 ? 2 : (C <= 16 ? 4 : (C <= 32 ? 8 : 16));
+    const int64_t blocks = ceil_div(M, kGatherThreads / lps);
+    TFH_REQUIRE(blocks < (1ll << 31), "vm gather: too many points (%lld)", (long long)M);
+#define TFH_G4(L)                                                                            \
+  hipLaunchKernelGGL((k_vm_gather4<Src, L>), dim3((unsigned)blocks), dim3(kGatherThreads), 0, \
+                     st, vm, src, M, out)
+    if (lps == 2) TFH_G4(2);
+    else if (lps == 4) TFH_G4(4);
+    else if (lps == 8) TFH_G4(8);
+    else TFH_G4(16);
+#undef TFH_G4
+    return check_launch("vm gather");
+  }
   const bool narrow = vm.channels <= 8;
   const int spb = kGatherThreads / (narrow ? 8 : 16);
   const int64_t blocks = ceil_div(M, spb);
