@@ -98,7 +98,6 @@ __global__ void __launch_bounds__(64) k_density_pass(tfh_vm dvm, RayIn in, Densi
   extern __shared__ float sm[];
   const int64_t r = blockIdx.x;
   const int lane = threadIdx.x;
-  const int grp = lane >> 4, l16 = lane & 15;
   const int N = in.N;
   float* s_a = sm;       // a_i = sigma_i * delta_i  (later: top-k keys)
   float* s_e = sm + N;   // E_i
@@ -111,9 +110,15 @@ __global__ void __launch_bounds__(64) k_density_pass(tfh_vm dvm, RayIn in, Densi
   const float a0x = in.aabb[0], a0y = in.aabb[1], a0z = in.aabb[2];
   const float a1x = in.aabb[3], a1y = in.aabb[4], a1z = in.aabb[5];
   const int G = (int)dvm.grid_dim, C = (int)dvm.channels;
+  // 16-byte taps when the channels allow: lps lanes per sample (C/4 rounded
+  // up to a power of two), 64/lps samples per step; else 16 lanes and
+  // 4-byte taps
+  const bool wide = in.wide_taps;
+  const int lps = !wide ? 16 : (C <= 8 ? 2 : (C <= 16 ? 4 : (C <= 32 ? 8 : 16)));
+  const int gs = lane / lps, ls = lane % lps;
 
-  for (int i0 = 0; i0 < N; i0 += 4) {
-    const int i = i0 + grp;
+  for (int i0 = 0; i0 < N; i0 += 64 / lps) {
+    const int i = i0 + gs;
     const bool act = i < N;
     float t = 0.f, dlt = 0.f, px = 0.f, py = 0.f, pz = 0.f;
     if (act) sample_point(in, r, i, ox, oy, oz, dx, dy, dz, t0, step, t, dlt, px, py, pz);
@@ -126,18 +131,30 @@ __global__ void __launch_bounds__(64) k_density_pass(tfh_vm dvm, RayIn in, Densi
     for (int m = 0; m < 3; ++m) {
       TapAddr a;
       tap_addrs(mt[m], m, G, a);
+      if (wide) {
+        if (act && 4 * ls < C) {
+          float4 lv, pv;
+          tap_values4(dvm.plane, dvm.line, C, 4 * ls, a, lv, pv);
+          acc += lv.x * pv.x;
+          acc += lv.y * pv.y;
+          acc += lv.z * pv.z;
+          acc += lv.w * pv.w;
+        }
+      } else {
 #pragma unroll
-      for (int q = 0; q < kMaxQ; ++q) {
-        const int c = l16 + 16 * q;
-        if (c < C && act) {
-          float lv, pv;
-          tap_values(dvm.plane, dvm.line, C, c, a, lv, pv);
-          acc += lv * pv;
+        for (int q = 0; q < kMaxQ; ++q) {
+          const int c = ls + 16 * q;
+          if (c < C && act) {
+            float lv, pv;
+            tap_values(dvm.plane, dvm.line, C, c, a, lv, pv);
+            acc += lv * pv;
+          }
         }
       }
     }
-    acc = group16_sum(acc);                        // sum over the 3C channels
-    if (act && l16 == 0) {
+    // sum over the 3C channels: the sample's lps lanes
+    for (int o = 1; o < lps; o <<= 1) acc += __shfl_xor(acc, o, 64);
+    if (act && ls == 0) {
       const float sig = softplus(acc + in.density_shift);  // render.py:207 (+10 there)
       s_a[i] = sig * dlt;                          // render.py:322
       const int64_t s = r * N + i;
@@ -645,8 +662,12 @@ static int32_t render_forward(const tfh_vm* density, const tfh_vm* appearance, c
   dout.out = out;
   {
   ProfScope ps("density_pass", st);
+  RayIn din = in;
+  din.wide_taps = density->channels % 4 == 0 && density->channels <= 64 &&
+                  (reinterpret_cast<uintptr_t>(density->plane) & 15) == 0 &&
+                  (reinterpret_cast<uintptr_t>(density->line) & 15) == 0;
   hipLaunchKernelGGL(k_density_pass, dim3((unsigned)R), dim3(64), 3 * N * sizeof(float), st,
-                     *density, in, dout);
+                     *density, din, dout);
   }
   TFH_TRY(check_launch("render: density pass"));
   if (cfg->mode != TFH_MODE_RGB) return TFH_OK;

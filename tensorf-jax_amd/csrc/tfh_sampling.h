@@ -61,6 +61,7 @@ struct RayIn {
   const float* base_ts;     // (N) contraction schedule, or (R, N+1) interval edges
   const float* base_steps;  // (N)
   const float* gumbel;      // (N) or null
+  int wide_taps = 0;        // density gather with 16-byte taps (set at launch)
 };
 
 // Sample i of ray r: distance t, step size delta and the world (or
