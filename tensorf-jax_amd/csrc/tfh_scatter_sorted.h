@@ -53,11 +53,21 @@ struct RowGrad {
   __device__ __forceinline__ float at(int64_t row, int m, int c) const {
     return g[row * 3 * C + m * C + c];
   }
+  // channels c4 .. c4+3 (16-byte aligned rows: C % 4 == 0)
+  __device__ __forceinline__ float4 at4(int64_t row, int m, int c4) const {
+    return *reinterpret_cast<const float4*>(g + row * 3 * C + m * C + c4);
+  }
+  bool aligned16() const { return (reinterpret_cast<uintptr_t>(g) & 15) == 0 && C % 4 == 0; }
 };
 
 struct ScalarGrad {
   const float* g;
   __device__ __forceinline__ float at(int64_t row, int, int) const { return g[row]; }
+  __device__ __forceinline__ float4 at4(int64_t row, int, int) const {
+    const float v = g[row];
+    return make_float4(v, v, v, v);
+  }
+  bool aligned16() const { return true; }
 };
 
 // Two-level sort of the (row, mode) pairs by plane cell. Cells are grouped
@@ -353,6 +363,109 @@ __global__ void __launch_bounds__(256) k_plane_scatter_sorted(tfh_vm vm, Src src
   if (cur >= 0) flush();
 }
 
+// Channel 16q + l of a 16-lane group whose lane j holds channels
+// 4j .. 4j+3 (v): the sums go back to one channel per lane before they are
+// added, so that each atomic instruction of a group covers 64 contiguous
+// bytes (one memory-side transaction) as in the scalar kernel; with the
+// quads the lanes' adds were 16 bytes apart, three transactions per
+// instruction (the plane pass took 1.54 ms instead of 0.69). All 16 lanes of
+// the group must call it (gl0: the group's first lane in the wave).
+__device__ __forceinline__ float quad_to_lane(const float (&v)[4], int q, int l, int gl0) {
+  const int from = gl0 + 4 * q + (l >> 2);
+  const float t0 = __shfl(v[0], from, 64), t1 = __shfl(v[1], from, 64);
+  const float t2 = __shfl(v[2], from, 64), t3 = __shfl(v[3], from, 64);
+  const int u = l & 3;
+  return u == 0 ? t0 : (u == 1 ? t1 : (u == 2 ? t2 : t3));
+}
+
+// k_plane_scatter_sorted with 16-byte loads: lane l of a 16-lane group of 64
+// pairs owns channels 4l .. 4l+3 (C % 4 == 0, 16 < C <= 64). The same sums
+// per channel, a third of the loads for 48 channels.
+template <class Src, class GS, int LPS>
+__global__ void __launch_bounds__(256) k_plane_scatter_sorted4(tfh_vm vm, Src src, int64_t pairs,
+                                                               const uint32_t* __restrict__ order,
+                                                               GS gsrc,
+                                                               float* __restrict__ gplane) {
+  static_assert(LPS == 16, "one channel per lane for the adds");
+  const int G = (int)vm.grid_dim, C = (int)vm.channels;
+  const int l = threadIdx.x % LPS;
+  const int gbase = threadIdx.x - l;  // first lane of the group in the block
+  const int64_t first = ((int64_t)blockIdx.x * (256 / LPS) + threadIdx.x / LPS) * kPairsPerGroup;
+  if (first >= pairs) return;  // whole group: no cross-group exchange below
+  const int64_t last = min(first + (int64_t)kPairsPerGroup, pairs);
+  const int c4 = 4 * l;
+  const bool lane_on = c4 < C;
+  float4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  int cur = -1;  // bin being summed
+  int tex[4] = {0, 0, 0, 0};
+  const int gl0 = gbase & 63;
+  auto flush = [&]() {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float v[4] = {acc[t].x, acc[t].y, acc[t].z, acc[t].w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (16 * q < C) {  // group-uniform
+          const float val = quad_to_lane(v, q, l, gl0);
+          const int c = 16 * q + l;
+          if (c < C && val != 0.0f) unsafeAtomicAdd(gplane + (size_t)tex[t] * C + c, val);
+        }
+      }
+      acc[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  for (int64_t b0 = first; b0 < last; b0 += LPS) {
+    // lane l fetches pair b0 + l: its packed index and the row's point
+    const int64_t e = b0 + l;
+    uint32_t pk = 0xffffffffu;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (e < last) {
+      pk = order[e];
+      src.get((int64_t)(pk >> 2), px, py, pz);
+    }
+    const int nb = (int)min((int64_t)LPS, last - b0);
+    for (int j = 0; j < nb; ++j) {
+      const uint32_t p = __shfl(pk, gbase + j, 64);
+      const float x = __shfl(px, gbase + j, 64), y = __shfl(py, gbase + j, 64),
+                  z = __shfl(pz, gbase + j, 64);
+      const int64_t row = (int64_t)(p >> 2);
+      const int m = (int)(p & 3u);
+      Tap tv, ta, tb;
+      one_mode_taps(x, y, z, G, m, tv, ta, tb);
+      const int bin = plane_bin(m, ta, tb, G);
+      if (bin != cur) {
+        if (cur >= 0) flush();
+        cur = bin;
+        const int base = m * G;
+        tex[0] = (base + ta.i0) * G + tb.i0;
+        tex[1] = (base + ta.i0) * G + tb.i1;
+        tex[2] = (base + ta.i1) * G + tb.i0;
+        tex[3] = (base + ta.i1) * G + tb.i1;
+      }
+      if (lane_on) {
+        const float w[4] = {ta.w0 * tb.w0, ta.w0 * tb.w1, ta.w1 * tb.w0, ta.w1 * tb.w1};
+        const float4 l0 = *reinterpret_cast<const float4*>(vm.line + (size_t)(m * G + tv.i0) * C + c4);
+        const float4 l1 = *reinterpret_cast<const float4*>(vm.line + (size_t)(m * G + tv.i1) * C + c4);
+        const float4 g4 = gsrc.at4(row, m, c4);
+        const float4 gl = make_float4(g4.x * (tv.w0 * l0.x + tv.w1 * l1.x),
+                                      g4.y * (tv.w0 * l0.y + tv.w1 * l1.y),
+                                      g4.z * (tv.w0 * l0.z + tv.w1 * l1.z),
+                                      g4.w * (tv.w0 * l0.w + tv.w1 * l1.w));
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          acc[t].x += w[t] * gl.x;
+          acc[t].y += w[t] * gl.y;
+          acc[t].z += w[t] * gl.z;
+          acc[t].w += w[t] * gl.w;
+        }
+      }
+    }
+  }
+  if (cur >= 0) flush();
+}
+
 // Line gradients in row order: dL/dline[m][v][c] += w_v * plane_m(c) *
 // gout[row][m*C + c]; 16-lane groups, equal line texels of the wave's four
 // rows (in either tap slot) added once. grad_line holds nlc copies; block b
@@ -480,14 +593,31 @@ inline int32_t scatter_sorted_pairs(const tfh_vm& vm, const Src& src, int64_t M,
                                     const char* plane_stage, const char* line_stage,
                                     hipStream_t st) {
   if (M <= 0) return TFH_OK;
+  // 16-byte loads in the plane pass when the channels and every tap source
+  // allow; only for fields of more than 16 channels: a 16-channel field's
+  // 16-lane groups read a tap in one instruction either way, and narrower
+  // groups would split its adds over several transactions
+  const bool wide = vm.channels % 4 == 0 && vm.channels <= 64 &&
+                    (reinterpret_cast<uintptr_t>(vm.plane) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(vm.line) & 15) == 0 && gsrc.aligned16();
   {
     ProfScope ps(plane_stage, st);
     const int64_t pairs = 3 * M;
     const int64_t groups = ceil_div(pairs, kPairsPerGroup);
-    hipLaunchKernelGGL((k_plane_scatter_sorted<Src, GS>), dim3((unsigned)ceil_div(groups, 16)),
-                       dim3(256), 0, st, vm, src, pairs, s.order, gsrc, grad.plane);
+    const int C = (int)vm.channels;
+    if (wide && C > 16) {
+      hipLaunchKernelGGL((k_plane_scatter_sorted4<Src, GS, 16>),
+                         dim3((unsigned)ceil_div(groups, 16)), dim3(256), 0, st, vm, src, pairs,
+                         s.order, gsrc, grad.plane);
+    } else {
+      hipLaunchKernelGGL((k_plane_scatter_sorted<Src, GS>), dim3((unsigned)ceil_div(groups, 16)),
+                         dim3(256), 0, st, vm, src, pairs, s.order, gsrc, grad.plane);
+    }
     TFH_TRY(check_launch("vm scatter: planes"));
   }
+  // the line pass keeps 4-byte taps: with 16-byte ones its merged adds need
+  // the sums moved back to one channel per lane, which cost more than the
+  // loads saved (0.83 -> 0.89 ms for 48 channels at 2.1 M rows)
   ProfScope ps(line_stage, st);
   hipLaunchKernelGGL((k_line_scatter<Src, GS>), dim3((unsigned)ceil_div(M, kGatherThreads / 16)),
                      dim3(kGatherThreads), 0, st, vm, src, M, gsrc, grad.line, nlc);
