@@ -58,6 +58,13 @@ def stage_model(stage, R, N, k, cd, ca, precision, n_params, prop=None):
         return "hbm", R * N * (2 * 72 * cd + 36), "GB/s", PEAK_HBM_GBS
     if stage == "appearance_scatter":
         return "hbm", M * (2 * 72 * ca + 12 + 4 * 3 * ca), "GB/s", PEAK_HBM_GBS
+    # every sample shaded on one grid: the compositing backward (+ the pair
+    # sort) and one scatter of both fields
+    if stage == "composite_bwd":
+        return "hbm", R * N * 36, "GB/s", PEAK_HBM_GBS
+    if stage == "factor_scatter":
+        return ("hbm", R * N * 2 * 72 * cd + M * (2 * 72 * ca + 12 + 4 * 3 * ca), "GB/s",
+                PEAK_HBM_GBS)
     if stage == "adam":
         return "hbm", n_params * 32, "GB/s", PEAK_HBM_GBS
     if prop is not None:
@@ -92,6 +99,13 @@ def compulsory_bytes(stage, R, N, k, cd, ca, G, n_params):
         return R * N * 28 + M * (24 + sel) + R * 28 + 2 * dfac
     if stage == "appearance_scatter":
         return M * (12 + sel + 4 * 3 * ca) + afac + 2 * afac
+    if stage == "composite_bwd":
+        # the scans' inputs and dL/ds, dL/dc out; the sorted pair order out
+        return R * N * (28 + 4 + 12) + M * 24 + R * 28
+    if stage == "factor_scatter":
+        # the pair order, points, dL/ds and the feature gradient in; both
+        # fields' factors read and their gradients read and written once
+        return R * N * (12 + 12 + 4) + M * 4 * 3 * ca + 3 * (dfac + afac)
     if stage == "adam":
         return n_params * 32
     return None

@@ -705,8 +705,13 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
   // every density sample, row r*N + i
   const RaySamples dsrc{ws.d.pts, nullptr, N, N};
   const bool dsorted = ws.ds && sorted_scatter_ok(*density, R * N, ws.sort);
+  // every sample shaded, one grid: one sort and one pass per scatter serve
+  // both fields, after the MLP backward (dL/dsigma waits in the workspace)
+  const RowGrad gsrc{ws.dfeat, (int)appearance->channels};
+  const bool fuse = dsorted && ws.d.idx == nullptr && sorted_scatter_ok(*appearance, M, ws.sort) &&
+                    two_field_scatter_ok(*density, *appearance, gsrc);
   {
-    ProfScope ps("composite_bwd_density_scatter", st);
+    ProfScope ps(fuse ? "composite_bwd" : "composite_bwd_density_scatter", st);
     tfh_vm_grad dg = *dgrad;
     if (ws.dlcopy) {
       TFH_TRY(zero_copies(ws.dlcopy, ws.dnlc * dln, st));
@@ -719,18 +724,31 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
                          *density, bi, ws.dc, dg, nlc, ws.ds);
       TFH_TRY(check_launch("render bwd: compositing"));
       TFH_TRY(sort_pairs((int)density->grid_dim, dsrc, R * N, ws.sort, "density_sort", st));
-      TFH_TRY(scatter_sorted_pairs(*density, dsrc, R * N, ScalarGrad{ws.ds}, dg, nlc, ws.sort,
-                                   "density_scatter_planes", "density_scatter_lines", st));
+      if (!fuse)
+        TFH_TRY(scatter_sorted_pairs(*density, dsrc, R * N, ScalarGrad{ws.ds}, dg, nlc, ws.sort,
+                                     "density_scatter_planes", "density_scatter_lines", st));
     } else {
       hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)R), dim3(64 * kCompBwdWaves),
                          5 * N * sizeof(float), st, *density, bi, ws.dc, dg, nlc,
                          (float*)nullptr);
       TFH_TRY(check_launch("render bwd: compositing"));
     }
-    if (ws.dlcopy) TFH_TRY(sum_copies(ws.dlcopy, ws.dnlc, dln, dgrad->line, st));
+    if (ws.dlcopy && !fuse) TFH_TRY(sum_copies(ws.dlcopy, ws.dnlc, dln, dgrad->line, st));
   }
   TFH_TRY(mlp_bwd_rows(mlp, ws.feats, dirs, cams, M, k, ws.dc, ws.dfeat, mgrad, cfg->mlp_precision,
                        ws.mlp_ws, ws.mlp_ws_bytes, st, ws.act, ws.act ? ws.rgb : nullptr));
+  if (fuse) {
+    ProfScope ps("factor_scatter", st);
+    if (ws.alcopy) TFH_TRY(zero_copies(ws.alcopy, ws.anlc * aln, st));
+    const FieldOut fa{*appearance, agrad->plane, ws.alcopy ? ws.alcopy : agrad->line,
+                      ws.alcopy ? ws.anlc : 1};
+    const FieldOut fd{*density, dgrad->plane, ws.dlcopy ? ws.dlcopy : dgrad->line,
+                      ws.dlcopy ? ws.dnlc : 1};
+    TFH_TRY(scatter_two_fields(dsrc, M, ws.sort, gsrc, fa, ScalarGrad{ws.ds}, fd, st));
+    if (ws.dlcopy) TFH_TRY(sum_copies(ws.dlcopy, ws.dnlc, dln, dgrad->line, st));
+    if (ws.alcopy) TFH_TRY(sum_copies(ws.alcopy, ws.anlc, aln, agrad->line, st));
+    return TFH_OK;
+  }
   RaySamples src{ws.d.pts, ws.d.idx, k, N};
   ProfScope ps("appearance_scatter", st);
   const bool sorted = sorted_scatter_ok(*appearance, M, ws.sort);
@@ -741,7 +759,6 @@ static int32_t render_backward(const tfh_vm* density, const tfh_vm* appearance,
     if (!shared)
       TFH_TRY(sort_pairs((int)appearance->grid_dim, src, M, ws.sort, "appearance_sort", st));
   }
-  const RowGrad gsrc{ws.dfeat, (int)appearance->channels};
   if (!ws.alcopy) {
     if (sorted)
       return scatter_sorted_pairs(*appearance, src, M, gsrc, *agrad, 1, ws.sort,

@@ -540,6 +540,225 @@ __global__ void __launch_bounds__(kGatherThreads) k_line_scatter(tfh_vm vm, Src 
   }
 }
 
+// ---- both fields in one pass -------------------------------------------------
+// When every sample is shaded and the density (CD <= 16 channels, upstream
+// gradient dL/dsigma per row) and appearance (16 < CA <= 64, C % 4 == 0,
+// upstream gradient the feature row) factors share the grid, the pairs, the
+// taps and the line merge plan are the same for both: one pass computes them
+// once and scatters both fields.
+struct FieldOut {
+  tfh_vm vm;
+  float* gplane;
+  float* gline;  // nlc copies of the line gradient
+  int nlc;
+};
+
+// The plane pass of both fields: k_plane_scatter_sorted4 for the appearance
+// channels (lane l: 4l .. 4l+3) beside the scalar density channels (lane l:
+// channel l), one bin change flushing both.
+template <class Src>
+__global__ void __launch_bounds__(256) k_plane_scatter_two(Src src, int64_t pairs,
+                                                           const uint32_t* __restrict__ order,
+                                                           RowGrad ga, FieldOut fa, ScalarGrad gd,
+                                                           FieldOut fd) {
+  constexpr int LPS = 16;
+  const int G = (int)fa.vm.grid_dim, CA = (int)fa.vm.channels, CD = (int)fd.vm.channels;
+  const int l = threadIdx.x & 15;
+  const int gbase = threadIdx.x & ~15;
+  const int gl0 = gbase & 63;
+  const int64_t first = ((int64_t)blockIdx.x * (256 / LPS) + threadIdx.x / LPS) * kPairsPerGroup;
+  if (first >= pairs) return;  // whole group: no cross-group exchange below
+  const int64_t last = min(first + (int64_t)kPairsPerGroup, pairs);
+  const int c4 = 4 * l;
+  const bool a_on = c4 < CA, d_on = l < CD;
+  float4 acc[4];
+  float accd[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    acc[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    accd[t] = 0.0f;
+  }
+  int cur = -1;
+  int tex[4] = {0, 0, 0, 0};
+  auto flush = [&]() {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float v[4] = {acc[t].x, acc[t].y, acc[t].z, acc[t].w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (16 * q < CA) {  // group-uniform
+          const float val = quad_to_lane(v, q, l, gl0);
+          const int c = 16 * q + l;
+          if (c < CA && val != 0.0f) unsafeAtomicAdd(fa.gplane + (size_t)tex[t] * CA + c, val);
+        }
+      }
+      if (d_on && accd[t] != 0.0f) unsafeAtomicAdd(fd.gplane + (size_t)tex[t] * CD + l, accd[t]);
+      acc[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+      accd[t] = 0.0f;
+    }
+  };
+  for (int64_t b0 = first; b0 < last; b0 += LPS) {
+    const int64_t e = b0 + l;
+    uint32_t pk = 0xffffffffu;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (e < last) {
+      pk = order[e];
+      src.get((int64_t)(pk >> 2), px, py, pz);
+    }
+    const int nb = (int)min((int64_t)LPS, last - b0);
+    for (int j = 0; j < nb; ++j) {
+      const uint32_t p = __shfl(pk, gbase + j, 64);
+      const float x = __shfl(px, gbase + j, 64), y = __shfl(py, gbase + j, 64),
+                  z = __shfl(pz, gbase + j, 64);
+      const int64_t row = (int64_t)(p >> 2);
+      const int m = (int)(p & 3u);
+      Tap tv, ta, tb;
+      one_mode_taps(x, y, z, G, m, tv, ta, tb);
+      const int bin = plane_bin(m, ta, tb, G);
+      if (bin != cur) {
+        if (cur >= 0) flush();
+        cur = bin;
+        const int base = m * G;
+        tex[0] = (base + ta.i0) * G + tb.i0;
+        tex[1] = (base + ta.i0) * G + tb.i1;
+        tex[2] = (base + ta.i1) * G + tb.i0;
+        tex[3] = (base + ta.i1) * G + tb.i1;
+      }
+      const float w[4] = {ta.w0 * tb.w0, ta.w0 * tb.w1, ta.w1 * tb.w0, ta.w1 * tb.w1};
+      if (a_on) {
+        const float4 l0 =
+            *reinterpret_cast<const float4*>(fa.vm.line + (size_t)(m * G + tv.i0) * CA + c4);
+        const float4 l1 =
+            *reinterpret_cast<const float4*>(fa.vm.line + (size_t)(m * G + tv.i1) * CA + c4);
+        const float4 g4 = ga.at4(row, m, c4);
+        const float4 gl = make_float4(g4.x * (tv.w0 * l0.x + tv.w1 * l1.x),
+                                      g4.y * (tv.w0 * l0.y + tv.w1 * l1.y),
+                                      g4.z * (tv.w0 * l0.z + tv.w1 * l1.z),
+                                      g4.w * (tv.w0 * l0.w + tv.w1 * l1.w));
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          acc[t].x += w[t] * gl.x;
+          acc[t].y += w[t] * gl.y;
+          acc[t].z += w[t] * gl.z;
+          acc[t].w += w[t] * gl.w;
+        }
+      }
+      if (d_on) {
+        const float lv = tv.w0 * fd.vm.line[(size_t)(m * G + tv.i0) * CD + l] +
+                         tv.w1 * fd.vm.line[(size_t)(m * G + tv.i1) * CD + l];
+        const float gl = gd.at(row, m, l) * lv;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) accd[t] += w[t] * gl;
+      }
+    }
+  }
+  if (cur >= 0) flush();
+}
+
+// The line pass of both fields: k_line_scatter's groups, taps and merge plan
+// once; channel slots 0 .. NQA-1 (16 channels each) of the appearance, then
+// the density's (CD <= 16).
+template <class Src>
+__global__ void __launch_bounds__(kGatherThreads) k_line_scatter_two(Src src, int64_t M,
+                                                                     RowGrad ga, FieldOut fa,
+                                                                     ScalarGrad gd, FieldOut fd) {
+  constexpr int LPS = 16, SPB = kGatherThreads / LPS;
+  const int grp = threadIdx.x / LPS, l = threadIdx.x % LPS;
+  const int64_t s0 = (int64_t)blockIdx.x * SPB;
+  if (s0 >= M) return;
+  const int64_t s = s0 + grp;
+  const bool active = s < M;
+  const int G = (int)fa.vm.grid_dim, CA = (int)fa.vm.channels, CD = (int)fd.vm.channels;
+  float* glA = fa.gline + (int64_t)(blockIdx.x % fa.nlc) * 3 * G * CA;
+  float* glD = fd.gline + (int64_t)(blockIdx.x % fd.nlc) * 3 * G * CD;
+  float x = 0.f, y = 0.f, z = 0.f;
+  if (active) src.get(s, x, y, z);
+  ModeTaps mt[3];
+  mode_taps(x, y, z, G, mt);
+  const int wg = grp & 3;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    TapAddr a;
+    tap_addrs(mt[m], m, G, a);
+    int k0 = active ? a.line0 : -1 - 2 * wg, k1 = active ? a.line1 : -2 - 2 * wg;
+    const bool fold = active && k1 == k0;
+    if (fold) k1 = INT_MIN + wg;
+    int src0[3], src1[3];
+    bool lead0 = active, lead1 = active && !fold;
+#pragma unroll
+    for (int oi = 0; oi < 3; ++oi) {
+      const int o = LPS * (oi + 1);
+      const bool lower = (wg ^ (oi + 1)) < wg;
+      const int p0 = __shfl_xor(k0, o, 64), p1 = __shfl_xor(k1, o, 64);
+      src0[oi] = p0 == k0 ? 0 : (p1 == k0 ? 1 : -1);
+      src1[oi] = p0 == k1 ? 0 : (p1 == k1 ? 1 : -1);
+      if (src0[oi] >= 0 && lower) lead0 = false;
+      if (src1[oi] >= 0 && lower) lead1 = false;
+    }
+    // slots 0..3 appearance (while 16 q < CA), slot 4 density
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const bool dens = q == 4;
+      const int C = dens ? CD : CA;
+      if (!dens && LPS * q >= CA) continue;  // wave-uniform
+      const int c = dens ? l : l + LPS * q;
+      const bool on = active && c < C;
+      float v0 = 0.f, v1 = 0.f;
+      if (on) {
+        float lv, pv;
+        tap_values(dens ? fd.vm.plane : fa.vm.plane, dens ? fd.vm.line : fa.vm.line, C, c, a, lv,
+                   pv);
+        const float gp = (dens ? gd.at(s, m, c) : ga.at(s, m, c)) * pv;
+        v0 = a.wl0 * gp;
+        v1 = a.wl1 * gp;
+        if (fold) {
+          v0 += v1;
+          v1 = 0.f;
+        }
+      }
+      float acc0 = v0, acc1 = v1;
+#pragma unroll
+      for (int oi = 0; oi < 3; ++oi) {
+        const int o = LPS * (oi + 1);
+        const float q0 = __shfl_xor(v0, o, 64), q1 = __shfl_xor(v1, o, 64);
+        acc0 += src0[oi] == 0 ? q0 : (src0[oi] == 1 ? q1 : 0.0f);
+        acc1 += src1[oi] == 0 ? q0 : (src1[oi] == 1 ? q1 : 0.0f);
+      }
+      float* gl = dens ? glD : glA;
+      if (lead0 && acc0 != 0.0f) unsafeAtomicAdd(gl + (size_t)k0 * C + c, acc0);
+      if (lead1 && acc1 != 0.0f) unsafeAtomicAdd(gl + (size_t)k1 * C + c, acc1);
+    }
+  }
+}
+
+// Whether the two fields can take the one-pass scatter.
+inline bool two_field_scatter_ok(const tfh_vm& den, const tfh_vm& app, const RowGrad& ga) {
+  return den.grid_dim == app.grid_dim && den.channels >= 1 && den.channels <= 16 &&
+         app.channels > 16 && app.channels <= 64 && app.channels % 4 == 0 &&
+         (reinterpret_cast<uintptr_t>(app.plane) & 15) == 0 &&
+         (reinterpret_cast<uintptr_t>(app.line) & 15) == 0 && ga.aligned16();
+}
+
+// Both fields' plane and line passes over the pairs sort_pairs ordered.
+template <class Src>
+inline int32_t scatter_two_fields(const Src& src, int64_t M, const SortWs& s, const RowGrad& ga,
+                                  const FieldOut& fa, const ScalarGrad& gd, const FieldOut& fd,
+                                  hipStream_t st) {
+  if (M <= 0) return TFH_OK;
+  {
+    ProfScope ps("factor_scatter_planes", st);
+    const int64_t pairs = 3 * M;
+    const int64_t groups = ceil_div(pairs, kPairsPerGroup);
+    hipLaunchKernelGGL((k_plane_scatter_two<Src>), dim3((unsigned)ceil_div(groups, 16)), dim3(256),
+                       0, st, src, pairs, s.order, ga, fa, gd, fd);
+    TFH_TRY(check_launch("vm scatter: planes of both fields"));
+  }
+  ProfScope ps("factor_scatter_lines", st);
+  hipLaunchKernelGGL((k_line_scatter_two<Src>), dim3((unsigned)ceil_div(M, kGatherThreads / 16)),
+                     dim3(kGatherThreads), 0, st, src, M, ga, fa, gd, fd);
+  return check_launch("vm scatter: lines of both fields");
+}
+
 // Bytes of the sort's workspace for up to M rows.
 inline size_t sort_ws_bytes(int64_t M) {
   return align_up((size_t)(3 * M) * 4, 256) * 2 + align_up((size_t)(3 * M) * 2, 256) +
