@@ -935,14 +935,28 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
     if constexpr (!ACT) forward_out<TM, NW, BF>(d, w, op, H2, Z3);
     if constexpr (ACT && FILM) {
       // the layer-3 input from the kept relu(z2): the embedding's scale and
-      // shift on the second half of the columns (forward_hidden's layer 2)
-      for (int e = threadIdx.x; e < TM * kU; e += blockDim.x) {
-        const int i = e / kU, c = e - i * kU;
+      // shift on the second half of the columns (forward_hidden's layer 2).
+      // A thread keeps one column (blockDim.x is a multiple of kU); when the
+      // tile's rows share a camera (a tile lies in one ray whenever rays
+      // have at least TM rows) its scale and shift are read once
+      static_assert((NW * 64) % kU == 0, "a thread keeps one column");
+      const int c = threadIdx.x % kU;
+      const int tcam = d.ncam > 0 ? Run[0] : -1;
+      float esc = 0.0f, esh = 0.0f;
+      if (d.ncam > 0 && c >= kU / 2 && tcam >= 0) {
+        esc = w.embed[(size_t)tcam * kU + c - kU / 2];
+        esh = w.embed[(size_t)tcam * kU + c];
+      }
+      for (int i = threadIdx.x / kU; i < TM; i += blockDim.x / kU) {
         const float h = H2p[i * d.ldU + c];
         float out = h;
         if (d.ncam > 0 && c >= kU / 2) {
-          const float* emb = w.embed + (size_t)Cs[i] * kU;
-          out = emb[c - kU / 2] * h + emb[c];
+          if (tcam >= 0) {
+            out = esc * h + esh;
+          } else {
+            const float* emb = w.embed + (size_t)Cs[i] * kU;
+            out = emb[c - kU / 2] * h + emb[c];
+          }
         }
         H2[i * d.ldU + c] = out;
       }
@@ -979,6 +993,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
       if constexpr (FILM) {
         const bool film = d.ncam > 0 && 16 * ct >= kU / 2;  // wave-uniform
         const int tcam = d.ncam > 0 ? Run[0] : -1;
+        // the tile camera's scale of this column, read once
+        const float tscale = (film && tcam >= 0) ? w.embed[(size_t)tcam * kU + col - kU / 2] : 0.0f;
         float sh = 0.0f, sg = 0.0f;
 #pragma unroll
         for (int q = 0; q < 4 * BRTI; ++q) {
@@ -997,7 +1013,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
                 unsafeAtomicAdd(gemb + col, g);
               }
             }
-            g = g * w.embed[(size_t)Cs[row] * kU + col - kU / 2];
+            g = g * (tcam >= 0 ? tscale : w.embed[(size_t)Cs[row] * kU + col - kU / 2]);
           }
           const float dz = hp > 0.0f ? g : 0.0f;
           H2[row * d.ldU + col] = dz;
