@@ -105,15 +105,17 @@ __device__ __forceinline__ void pair_tile(int m, const Tap& a, const Tap& b, int
 }
 
 // (A) Per workgroup of kRowsPerRankGroup rows: LDS ranks within tiles, one
-// global reservation per tile present.
-template <class Src>
+// global reservation per tile present. LINE: the bins are instead the line
+// cells of each mode (m (g+1) + the line tap code, 3 (g+1) bins), with no
+// cell pass after.
+template <class Src, bool LINE = false>
 __global__ void __launch_bounds__(256) k_tile_rank(Src src, int64_t M, int g,
                                                    int32_t* __restrict__ tile_count,
                                                    uint16_t* __restrict__ tile_of,
                                                    uint32_t* __restrict__ rank_of) {
   constexpr int RPT = kRowsPerRankGroup / 256;
   __shared__ int32_t cnt[kSortMaxTiles];
-  const int nt = tiles_per_axis(g), tiles = 3 * nt * nt;
+  const int nt = tiles_per_axis(g), tiles = LINE ? 3 * (g + 1) : 3 * nt * nt;
   for (int t = threadIdx.x; t < tiles; t += 256) cnt[t] = 0;
   __syncthreads();
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerRankGroup;
@@ -130,8 +132,12 @@ __global__ void __launch_bounds__(256) k_tile_rank(Src src, int64_t M, int g,
       mode_taps(x, y, z, g, mt);
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        int cell;
-        pair_tile(m, mt[m].a, mt[m].b, g, tl[j][m], cell);
+        if constexpr (LINE) {
+          tl[j][m] = m * (g + 1) + tap_code(mt[m].v, g);
+        } else {
+          int cell;
+          pair_tile(m, mt[m].a, mt[m].b, g, tl[j][m], cell);
+        }
         rk[j][m] = atomicAdd(&cnt[tl[j][m]], 1);
       }
     }
@@ -731,6 +737,128 @@ __global__ void __launch_bounds__(kGatherThreads) k_line_scatter_two(Src src, in
   }
 }
 
+// The line pass of both fields over pairs ordered by line cell (sort_lines):
+// a 16-lane group walks 64 pairs, sums the two line taps' gradients of a
+// cell's run in registers (appearance: 16-byte plane taps, lane l channels
+// 4l .. 4l+3; density: lane l channel l) and adds them when the cell
+// changes. A line texel collects the adds of every sample on its plane
+// slice, so in ray order it draws one memory-side transaction per sample
+// and 16 channels wherever consecutive samples of a ray move to a new
+// texel; in cell order it draws a few per run of thousands of pairs. The
+// adds go straight to the line gradients (no copies).
+static_assert(3 * (kSortMaxGrid + 1) <= kSortMaxTiles, "line bins fit the sort's tile table");
+
+template <class Src>
+__global__ void __launch_bounds__(256) k_line_walk_two(Src src, int64_t pairs,
+                                                       const uint32_t* __restrict__ order,
+                                                       RowGrad ga, FieldOut fa, ScalarGrad gd,
+                                                       FieldOut fd) {
+  constexpr int LPS = 16;
+  const int G = (int)fa.vm.grid_dim, CA = (int)fa.vm.channels, CD = (int)fd.vm.channels;
+  const int l = threadIdx.x & 15;
+  const int gbase = threadIdx.x & ~15;
+  const int gl0 = gbase & 63;
+  const int64_t first = ((int64_t)blockIdx.x * (256 / LPS) + threadIdx.x / LPS) * kPairsPerGroup;
+  if (first >= pairs) return;  // whole group: no cross-group exchange below
+  const int64_t last = min(first + (int64_t)kPairsPerGroup, pairs);
+  const int c4 = 4 * l;
+  const bool a_on = c4 < CA, d_on = l < CD;
+  float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
+  float d0 = 0.0f, d1 = 0.0f;
+  int cur = -1, cell0 = 0, cell1 = 0;
+  auto flush = [&]() {
+    const float v0[4] = {a0.x, a0.y, a0.z, a0.w}, v1[4] = {a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (16 * q < CA) {  // group-uniform
+        const float x0 = quad_to_lane(v0, q, l, gl0), x1 = quad_to_lane(v1, q, l, gl0);
+        const int c = 16 * q + l;
+        if (c < CA) {
+          if (x0 != 0.0f) unsafeAtomicAdd(fa.gline + (size_t)cell0 * CA + c, x0);
+          if (x1 != 0.0f) unsafeAtomicAdd(fa.gline + (size_t)cell1 * CA + c, x1);
+        }
+      }
+    }
+    if (d_on) {
+      if (d0 != 0.0f) unsafeAtomicAdd(fd.gline + (size_t)cell0 * CD + l, d0);
+      if (d1 != 0.0f) unsafeAtomicAdd(fd.gline + (size_t)cell1 * CD + l, d1);
+    }
+    a0 = a1 = make_float4(0.f, 0.f, 0.f, 0.f);
+    d0 = d1 = 0.0f;
+  };
+  for (int64_t b0 = first; b0 < last; b0 += LPS) {
+    const int64_t e = b0 + l;
+    uint32_t pk = 0xffffffffu;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (e < last) {
+      pk = order[e];
+      src.get((int64_t)(pk >> 2), px, py, pz);
+    }
+    const int nb = (int)min((int64_t)LPS, last - b0);
+    for (int j = 0; j < nb; ++j) {
+      const uint32_t p = __shfl(pk, gbase + j, 64);
+      const float x = __shfl(px, gbase + j, 64), y = __shfl(py, gbase + j, 64),
+                  z = __shfl(pz, gbase + j, 64);
+      const int64_t row = (int64_t)(p >> 2);
+      const int m = (int)(p & 3u);
+      Tap tv, ta, tb;
+      one_mode_taps(x, y, z, G, m, tv, ta, tb);
+      const int bin = m * (G + 1) + tap_code(tv, G);
+      if (bin != cur) {
+        if (cur >= 0) flush();
+        cur = bin;
+        cell0 = m * G + tv.i0;
+        cell1 = m * G + tv.i1;
+      }
+      // the four plane taps of mode m (tap_addrs' order and weights)
+      const int base = m * G;
+      const int64_t p00 = (int64_t)(base + ta.i0) * G + tb.i0, p01 = (int64_t)(base + ta.i0) * G + tb.i1;
+      const int64_t p10 = (int64_t)(base + ta.i1) * G + tb.i0, p11 = (int64_t)(base + ta.i1) * G + tb.i1;
+      const float w00 = ta.w0 * tb.w0, w01 = ta.w0 * tb.w1, w10 = ta.w1 * tb.w0, w11 = ta.w1 * tb.w1;
+      if (a_on) {
+        const float* pl = fa.vm.plane + c4;
+        const float4 t00 = *reinterpret_cast<const float4*>(pl + p00 * CA);
+        const float4 t01 = *reinterpret_cast<const float4*>(pl + p01 * CA);
+        const float4 t10 = *reinterpret_cast<const float4*>(pl + p10 * CA);
+        const float4 t11 = *reinterpret_cast<const float4*>(pl + p11 * CA);
+        const float4 g4 = ga.at4(row, m, c4);
+        const float gx = g4.x * (((w00 * t00.x + w01 * t01.x) + w10 * t10.x) + w11 * t11.x);
+        const float gy = g4.y * (((w00 * t00.y + w01 * t01.y) + w10 * t10.y) + w11 * t11.y);
+        const float gz = g4.z * (((w00 * t00.z + w01 * t01.z) + w10 * t10.z) + w11 * t11.z);
+        const float gw = g4.w * (((w00 * t00.w + w01 * t01.w) + w10 * t10.w) + w11 * t11.w);
+        a0.x += tv.w0 * gx; a0.y += tv.w0 * gy; a0.z += tv.w0 * gz; a0.w += tv.w0 * gw;
+        a1.x += tv.w1 * gx; a1.y += tv.w1 * gy; a1.z += tv.w1 * gz; a1.w += tv.w1 * gw;
+      }
+      if (d_on) {
+        const float* pl = fd.vm.plane + l;
+        const float pv = ((w00 * pl[p00 * CD] + w01 * pl[p01 * CD]) + w10 * pl[p10 * CD]) +
+                         w11 * pl[p11 * CD];
+        const float gp = gd.at(row, m, l) * pv;
+        d0 += tv.w0 * gp;
+        d1 += tv.w1 * gp;
+      }
+    }
+  }
+  if (cur >= 0) flush();
+}
+
+// Orders the (row, mode) pairs by line cell (3 (g+1) bins: rank, scan,
+// place; the bin is the whole key).
+template <class Src>
+inline int32_t sort_lines(int g, const Src& src, int64_t M, const SortWs& s, hipStream_t st) {
+  const int bins = 3 * (g + 1);
+  const int64_t pairs = 3 * M;
+  ProfScope ps("line_sort", st);
+  TFH_TRY(zero_copies(reinterpret_cast<float*>(s.tile_count), bins, st));  // all-zero bits
+  hipLaunchKernelGGL((k_tile_rank<Src, true>), dim3((unsigned)ceil_div(M, kRowsPerRankGroup)),
+                     dim3(256), 0, st, src, M, g, s.tile_count, s.tile_of, s.rank_of);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, s.tile_count, bins, s.tile_start,
+                     s.chunk_start);
+  hipLaunchKernelGGL(k_tile_place, dim3((unsigned)ceil_div(pairs, 256)), dim3(256), 0, st, pairs,
+                     s.tile_of, s.rank_of, s.tile_start, s.order);
+  return check_launch("vm scatter: line sort");
+}
+
 // Whether the two fields can take the one-pass scatter.
 inline bool two_field_scatter_ok(const tfh_vm& den, const tfh_vm& app, const RowGrad& ga) {
   return den.grid_dim == app.grid_dim && den.channels >= 1 && den.channels <= 16 &&
@@ -740,18 +868,35 @@ inline bool two_field_scatter_ok(const tfh_vm& den, const tfh_vm& app, const Row
 }
 
 // Both fields' plane and line passes over the pairs sort_pairs ordered.
+// lines_sorted: the line pass on pairs re-sorted by line cell (the sort's
+// arrays are reused after the plane pass), adding straight into lines_a /
+// lines_d (the fields' line gradients); else the ray-order pass into fa's
+// and fd's line copies.
 template <class Src>
 inline int32_t scatter_two_fields(const Src& src, int64_t M, const SortWs& s, const RowGrad& ga,
                                   const FieldOut& fa, const ScalarGrad& gd, const FieldOut& fd,
-                                  hipStream_t st) {
+                                  hipStream_t st, bool lines_sorted = false,
+                                  float* lines_a = nullptr, float* lines_d = nullptr) {
   if (M <= 0) return TFH_OK;
+  const int64_t pairs = 3 * M;
+  const int64_t groups = ceil_div(pairs, kPairsPerGroup);
   {
     ProfScope ps("factor_scatter_planes", st);
-    const int64_t pairs = 3 * M;
-    const int64_t groups = ceil_div(pairs, kPairsPerGroup);
     hipLaunchKernelGGL((k_plane_scatter_two<Src>), dim3((unsigned)ceil_div(groups, 16)), dim3(256),
                        0, st, src, pairs, s.order, ga, fa, gd, fd);
     TFH_TRY(check_launch("vm scatter: planes of both fields"));
+  }
+  if (lines_sorted) {
+    TFH_TRY(sort_lines((int)fa.vm.grid_dim, src, M, s, st));
+    ProfScope ps("factor_scatter_lines", st);
+    FieldOut la = fa, ld = fd;
+    la.gline = lines_a;
+    la.nlc = 1;
+    ld.gline = lines_d;
+    ld.nlc = 1;
+    hipLaunchKernelGGL((k_line_walk_two<Src>), dim3((unsigned)ceil_div(groups, 16)), dim3(256), 0,
+                       st, src, pairs, s.order, ga, la, gd, ld);
+    return check_launch("vm scatter: sorted lines of both fields");
   }
   ProfScope ps("factor_scatter_lines", st);
   hipLaunchKernelGGL((k_line_scatter_two<Src>), dim3((unsigned)ceil_div(M, kGatherThreads / 16)),
