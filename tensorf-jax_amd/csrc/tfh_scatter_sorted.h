@@ -620,6 +620,20 @@ __global__ void __launch_bounds__(256) k_plane_scatter_two(Src src, int64_t pair
       const int m = (int)(p & 3u);
       Tap tv, ta, tb;
       one_mode_taps(x, y, z, G, m, tv, ta, tb);
+      // both fields' line taps and upstream gradients requested before the
+      // flush: one memory round trip per pair instead of one per field
+      float4 l0 = make_float4(0.f, 0.f, 0.f, 0.f), l1 = l0, g4 = l0;
+      float dl0 = 0.f, dl1 = 0.f, dg = 0.f;
+      if (a_on) {
+        l0 = *reinterpret_cast<const float4*>(fa.vm.line + (size_t)(m * G + tv.i0) * CA + c4);
+        l1 = *reinterpret_cast<const float4*>(fa.vm.line + (size_t)(m * G + tv.i1) * CA + c4);
+        g4 = ga.at4(row, m, c4);
+      }
+      if (d_on) {
+        dl0 = fd.vm.line[(size_t)(m * G + tv.i0) * CD + l];
+        dl1 = fd.vm.line[(size_t)(m * G + tv.i1) * CD + l];
+        dg = gd.at(row, m, l);
+      }
       const int bin = plane_bin(m, ta, tb, G);
       if (bin != cur) {
         if (cur >= 0) flush();
@@ -632,11 +646,6 @@ __global__ void __launch_bounds__(256) k_plane_scatter_two(Src src, int64_t pair
       }
       const float w[4] = {ta.w0 * tb.w0, ta.w0 * tb.w1, ta.w1 * tb.w0, ta.w1 * tb.w1};
       if (a_on) {
-        const float4 l0 =
-            *reinterpret_cast<const float4*>(fa.vm.line + (size_t)(m * G + tv.i0) * CA + c4);
-        const float4 l1 =
-            *reinterpret_cast<const float4*>(fa.vm.line + (size_t)(m * G + tv.i1) * CA + c4);
-        const float4 g4 = ga.at4(row, m, c4);
         const float4 gl = make_float4(g4.x * (tv.w0 * l0.x + tv.w1 * l1.x),
                                       g4.y * (tv.w0 * l0.y + tv.w1 * l1.y),
                                       g4.z * (tv.w0 * l0.z + tv.w1 * l1.z),
@@ -650,9 +659,7 @@ __global__ void __launch_bounds__(256) k_plane_scatter_two(Src src, int64_t pair
         }
       }
       if (d_on) {
-        const float lv = tv.w0 * fd.vm.line[(size_t)(m * G + tv.i0) * CD + l] +
-                         tv.w1 * fd.vm.line[(size_t)(m * G + tv.i1) * CD + l];
-        const float gl = gd.at(row, m, l) * lv;
+        const float gl = dg * (tv.w0 * dl0 + tv.w1 * dl1);
 #pragma unroll
         for (int t = 0; t < 4; ++t) accd[t] += w[t] * gl;
       }
@@ -803,6 +810,29 @@ __global__ void __launch_bounds__(256) k_line_walk_two(Src src, int64_t pairs,
       const int m = (int)(p & 3u);
       Tap tv, ta, tb;
       one_mode_taps(x, y, z, G, m, tv, ta, tb);
+      // the four plane taps of mode m (tap_addrs' order and weights) and the
+      // upstream gradients of both fields, requested before the flush
+      const int base = m * G;
+      const int64_t p00 = (int64_t)(base + ta.i0) * G + tb.i0, p01 = (int64_t)(base + ta.i0) * G + tb.i1;
+      const int64_t p10 = (int64_t)(base + ta.i1) * G + tb.i0, p11 = (int64_t)(base + ta.i1) * G + tb.i1;
+      float4 t00 = make_float4(0.f, 0.f, 0.f, 0.f), t01 = t00, t10 = t00, t11 = t00, g4 = t00;
+      float d00 = 0.f, d01 = 0.f, d10 = 0.f, d11 = 0.f, dg = 0.f;
+      if (a_on) {
+        const float* pl = fa.vm.plane + c4;
+        t00 = *reinterpret_cast<const float4*>(pl + p00 * CA);
+        t01 = *reinterpret_cast<const float4*>(pl + p01 * CA);
+        t10 = *reinterpret_cast<const float4*>(pl + p10 * CA);
+        t11 = *reinterpret_cast<const float4*>(pl + p11 * CA);
+        g4 = ga.at4(row, m, c4);
+      }
+      if (d_on) {
+        const float* pl = fd.vm.plane + l;
+        d00 = pl[p00 * CD];
+        d01 = pl[p01 * CD];
+        d10 = pl[p10 * CD];
+        d11 = pl[p11 * CD];
+        dg = gd.at(row, m, l);
+      }
       const int bin = m * (G + 1) + tap_code(tv, G);
       if (bin != cur) {
         if (cur >= 0) flush();
@@ -810,18 +840,8 @@ __global__ void __launch_bounds__(256) k_line_walk_two(Src src, int64_t pairs,
         cell0 = m * G + tv.i0;
         cell1 = m * G + tv.i1;
       }
-      // the four plane taps of mode m (tap_addrs' order and weights)
-      const int base = m * G;
-      const int64_t p00 = (int64_t)(base + ta.i0) * G + tb.i0, p01 = (int64_t)(base + ta.i0) * G + tb.i1;
-      const int64_t p10 = (int64_t)(base + ta.i1) * G + tb.i0, p11 = (int64_t)(base + ta.i1) * G + tb.i1;
       const float w00 = ta.w0 * tb.w0, w01 = ta.w0 * tb.w1, w10 = ta.w1 * tb.w0, w11 = ta.w1 * tb.w1;
       if (a_on) {
-        const float* pl = fa.vm.plane + c4;
-        const float4 t00 = *reinterpret_cast<const float4*>(pl + p00 * CA);
-        const float4 t01 = *reinterpret_cast<const float4*>(pl + p01 * CA);
-        const float4 t10 = *reinterpret_cast<const float4*>(pl + p10 * CA);
-        const float4 t11 = *reinterpret_cast<const float4*>(pl + p11 * CA);
-        const float4 g4 = ga.at4(row, m, c4);
         const float gx = g4.x * (((w00 * t00.x + w01 * t01.x) + w10 * t10.x) + w11 * t11.x);
         const float gy = g4.y * (((w00 * t00.y + w01 * t01.y) + w10 * t10.y) + w11 * t11.y);
         const float gz = g4.z * (((w00 * t00.z + w01 * t01.z) + w10 * t10.z) + w11 * t11.z);
@@ -830,10 +850,8 @@ __global__ void __launch_bounds__(256) k_line_walk_two(Src src, int64_t pairs,
         a1.x += tv.w1 * gx; a1.y += tv.w1 * gy; a1.z += tv.w1 * gz; a1.w += tv.w1 * gw;
       }
       if (d_on) {
-        const float* pl = fd.vm.plane + l;
-        const float pv = ((w00 * pl[p00 * CD] + w01 * pl[p01 * CD]) + w10 * pl[p10 * CD]) +
-                         w11 * pl[p11 * CD];
-        const float gp = gd.at(row, m, l) * pv;
+        const float pv = ((w00 * d00 + w01 * d01) + w10 * d10) + w11 * d11;
+        const float gp = dg * pv;
         d0 += tv.w0 * gp;
         d1 += tv.w1 * gp;
       }
