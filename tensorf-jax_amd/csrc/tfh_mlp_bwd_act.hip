@@ -16,11 +16,11 @@ void launch_mlp_bwd_std_act(bool bf, unsigned nblk, size_t lds, hipStream_t st,
   if (bf) {
     allow_lds(k_mlp_bwd_kept<true>, lds);
     hipLaunchKernelGGL(k_mlp_bwd_kept<true>, dim3(nblk), dim3(kKeptNW * 64), lds, st, w, pk,
-                       feats, dirs, grgb, M, rpr, gfeat, partials, act, rgb);
+                       feats, dirs, grgb, M, rpr, gfeat, partials, act, rgb, nullptr, nullptr, 0);
   } else {
     allow_lds(k_mlp_bwd_kept<false>, lds);
     hipLaunchKernelGGL(k_mlp_bwd_kept<false>, dim3(nblk), dim3(kKeptNW * 64), lds, st, w, pk,
-                       feats, dirs, grgb, M, rpr, gfeat, partials, act, rgb);
+                       feats, dirs, grgb, M, rpr, gfeat, partials, act, rgb, nullptr, nullptr, 0);
   }
 }
 

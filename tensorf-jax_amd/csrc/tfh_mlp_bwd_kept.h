@@ -1,7 +1,8 @@
 // Backward of the standard FeatureMlp shape (networks.py:38-121; 144
-// features, squash 27, 2+2 frequencies, no camera embedding) that reads the
-// activations the forward kernel kept (relu(z1), the layer-3 input and the
-// output): the fused training steps' backward.
+// features, squash 27, 2+2 frequencies) that reads the activations the
+// forward kernel kept (relu(z1), the layer-3 input -- relu(z2) before the
+// FiLM camera embedding when there is one -- and the output): the fused
+// training steps' backward.
 //
 // One workgroup (8 waves) per CU walks tiles of 32 rows; the weight gradients
 // of dW1 / dW2 stay in registers of their owning wave and those of dW3 / dW0
@@ -19,7 +20,13 @@
 //   T5      dW1 ; dX = dZ1 @ W1^T (the columns that feed h0) over H2 / Z3
 //   T6      dH0 from dX and the sin / cos the encoding left in X
 //   T7      dW0 ; dF = dH0 @ W0^T
-// with one barrier after each step: 8 per tile.
+// with one barrier after each step: 8 per tile. With the camera embedding
+// (FILM, networks.py:103-111) the kept relu(z2) goes to its own region H2p,
+// T1 forms the layer-3 input H2 = e[:U/2] * h2 + e[U/2:] on the second half
+// of the columns (e = embed[cam]; read once per tile when its rows share a
+// camera), T2 runs dW3 beside the dZ2 strip, whose epilogue also takes the
+// embedding's gradient and leaves dZ2 in H2p, and the dX strip reads its B
+// table from L2 (the LDS copy gives way to H2p).
 #pragma once
 
 #include "tfh_mlp_kernels.h"
@@ -40,6 +47,7 @@ struct KeptRegs {
   i32x4 vf[kF];
   i32x4 va[kA];
   float s, g, dv;
+  int cam;  // camera of row threadIdx.x (< TM), with an embedding
 };
 
 // Issues the loads of the tile at row0 (features, view directions, kept
@@ -47,13 +55,14 @@ struct KeptRegs {
 // resource over the tile's valid rows, so rows past M read 0 (all of them
 // for a tile past the end: the loop issues one unconditionally, which keeps
 // the registers of r dead between commit and the next issue).
-template <int TM, int NT, bool BF>
+template <int TM, int NT, bool BF, bool FILM = false>
 __device__ __forceinline__ void kept_issue(const MlpDims& d, const float* __restrict__ feats,
                                            const float* __restrict__ dirs,
                                            const float* __restrict__ act,
                                            const float* __restrict__ rgb,
                                            const float* __restrict__ grgb, int64_t row0,
-                                           int64_t M, int64_t rpr, KeptRegs<TM, NT, BF>& r) {
+                                           int64_t M, int64_t rpr, KeptRegs<TM, NT, BF>& r,
+                                           const uint32_t* __restrict__ cams = nullptr) {
   const int nv = (int)max((int64_t)0, min((int64_t)TM, M - row0));
   const int64_t ray0 = row0 / rpr;
   const unsigned rem0 = (unsigned)(row0 - ray0 * rpr);
@@ -84,6 +93,11 @@ __device__ __forceinline__ void kept_issue(const MlpDims& d, const float* __rest
   const unsigned rel = (rem0 + (unsigned)i3) / (unsigned)rpr;  // ray of row i3 after ray0
   r.dv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
       rd, (own3 && i3 < nv) ? (int)(rel * 3 + c3) * 4 : kOOB, 0, 0));
+  if constexpr (FILM) {
+    const auto rc = buffer_rsrc(cams + ray0, nray * 4);
+    const unsigned relc = (rem0 + (unsigned)t) / (unsigned)rpr;  // ray of row t after ray0
+    r.cam = (int)__builtin_amdgcn_raw_buffer_load_b32(rc, t < nv ? (int)relc * 4 : kOOB, 0, 0);
+  }
   // the loads stay here, a phase ahead of their use: nothing moves a memory
   // access across this statement
   asm volatile("" ::: "memory");
@@ -91,10 +105,13 @@ __device__ __forceinline__ void kept_issue(const MlpDims& d, const float* __rest
 
 // Stores the issued tile: features (Fs, zero padded to FP), directions (Ds),
 // relu(z1) (H1), the layer-3 input (H2) and dZ3 (Z3, zero past the colours).
-template <int TM, int NT, bool BF>
+// With FILM the second half (relu(z2) before the embedding) goes to H2
+// (the caller passes H2p) and each row's camera index, clamped to
+// [0, ncam), to Cs.
+template <int TM, int NT, bool BF, bool FILM = false>
 __device__ __forceinline__ void kept_commit(const MlpDims& d, const KeptRegs<TM, NT, BF>& r,
                                             float* Fs, float* Ds, float* H1, float* H2,
-                                            float* Z3) {
+                                            float* Z3, int* Cs = nullptr, int ncam = 0) {
   const int fp4 = d.FP / 4, n4 = TM * fp4;
   const int t = threadIdx.x;
 #pragma unroll
@@ -127,14 +144,19 @@ __device__ __forceinline__ void kept_commit(const MlpDims& d, const KeptRegs<TM,
     const int i = e / (d.Z3W - 3), c = 3 + e % (d.Z3W - 3);
     Z3[i * d.ld3 + c] = 0.0f;
   }
+  if constexpr (FILM) {
+    if (t < TM) Cs[t] = (r.cam < 0 || r.cam >= ncam) ? 0 : r.cam;
+  }
 }
 
-template <bool BF>
+// FILM: cams (per ray), gembed (ncam x U, added to) and ncam >= 1.
+template <bool BF, bool FILM = false>
 __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
     MlpW w, MlpPack pk, const float* __restrict__ feats, const float* __restrict__ dirs,
     const float* __restrict__ grgb, int64_t M, int64_t rpr, float* __restrict__ gfeat,
     float* __restrict__ partials, const float* __restrict__ act,
-    const float* __restrict__ rgb_in) {
+    const float* __restrict__ rgb_in, const uint32_t* __restrict__ cams,
+    float* __restrict__ gembed, int ncam) {
   constexpr int TM = kKeptTM, NW = kKeptNW, NT = NW * 64, RT = TM / 16;
   constexpr MlpDims d = std_dims<BF>();
   constexpr GradPack gp = make_pack(d);
@@ -145,11 +167,20 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
   float* X = Fs + TM * d.ldF;       // TM x ldE
   float* H0 = X + TM * d.ldE;       // TM x ldS: layer-0 part B, later dH0
   float* H1 = H0 + TM * d.ldS;      // TM x ldU: relu(z1), later dZ1
-  float* H2 = H1 + TM * d.ldU;      // TM x ldU: layer-3 input, later dZ2, then dX
-  float* Z3 = H2 + TM * d.ldU;      // TM x ld3: dZ3
+  float* H2 = H1 + TM * d.ldU;      // TM x ldU: layer-3 input, later dZ2 (no FILM), then dX
+  // FILM: relu(z2) before the embedding, later dZ2 (without FILM: H2 itself)
+  float* H2p = FILM ? H2 + TM * d.ldU : H2;
+  float* Z3 = H2p + TM * d.ldU;     // TM x ld3: dZ3
   float* Ds = Z3 + TM * d.ld3;      // TM x 4
-  float* Wl = Ds + TM * 4 + TM * 4;  // weight-gradient LDS slots (after the unused camera ids)
-  float* DX = H2;                   // TM x kKeptLdDX
+  int* Cs = reinterpret_cast<int*>(Ds + TM * 4);  // TM x 4: camera of each row (FILM)
+  // FILM: running embedding gradient of camera Run[1] (g * h2 for the scale
+  // half, g for the shift half), added to gembed when the camera changes and
+  // at the end; Run[0]: the tile's camera when its rows share one, else -1
+  float* Eacc = reinterpret_cast<float*>(Cs + TM * 4);
+  int* Run = reinterpret_cast<int*>(Eacc + kU);
+  float* Wl = FILM ? Eacc + kU + 4 : Eacc;  // weight-gradient LDS slots
+  float* DX = H2;                   // TM x kKeptLdDX (over H2 and the dead dZ2 / dZ3 rows)
+  float* DZ2 = H2p;
   const Operands<BF> op{d, w, pk};
   const int wave = (threadIdx.x >> 6) & (NW - 1);  // < NW: slot tile offsets fold to constants
   const int lane = lane_id();
@@ -163,10 +194,15 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
   // t < 3*TM over its (row t/3, colour t%3) slot; reduced at the end
   float db1 = 0.f, db2 = 0.f, db3 = 0.f;
   for (int e = threadIdx.x; e < kLdsSlots * NW * 256; e += blockDim.x) Wl[e] = 0.0f;
-  // the first rows of the dX strip's B table in LDS (as k_mlp_bwd)
+  if constexpr (FILM) {
+    for (int c = threadIdx.x; c < kU; c += blockDim.x) Eacc[c] = 0.0f;
+    if (threadIdx.x == 0) Run[1] = -1;
+  }
+  // the first rows of the dX strip's B table in LDS (as k_mlp_bwd; not with FILM)
   __bf16* G1s = reinterpret_cast<__bf16*>(Wl + kLdsSlots * NW * 256);
   float* G1f = Wl + kLdsSlots * NW * 256;
-  if constexpr (BF) {
+  if constexpr (FILM) {
+  } else if constexpr (BF) {
     for (int e = threadIdx.x; e < kG1Rows * (kU / 8); e += blockDim.x) {
       const int j = e / (kU / 8), c8 = e - j * (kU / 8);
       *reinterpret_cast<uint4*>(G1s + j * kG1Ld + 8 * c8) =
@@ -186,11 +222,12 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
 
   const int64_t ntiles = (M + TM - 1) / TM;
   KeptRegs<TM, NT, BF> nx;
-  kept_issue<TM, NT, BF>(d, feats, dirs, act, rgb_in, grgb, (int64_t)blockIdx.x * TM, M, rpr, nx);
+  kept_issue<TM, NT, BF, FILM>(d, feats, dirs, act, rgb_in, grgb, (int64_t)blockIdx.x * TM, M, rpr,
+                               nx, cams);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t row0 = tile * TM;
     __syncthreads();
-    kept_commit<TM, NT, BF>(d, nx, Fs, Ds, H1, H2, Z3);
+    kept_commit<TM, NT, BF, FILM>(d, nx, Fs, Ds, H1, H2p, Z3, Cs, ncam);
     // the layer-0 item's B chunks (item wave: K part wave & 1, column tile
     // (wave >> 1) & 1), requested before the barrier
     constexpr int K0A = BF ? 96 : 80;
@@ -237,9 +274,52 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
         if (col < d.S) dst[row * ld + col] = acc[0][e];
       }
     }
-    wgrad_phase<TM, NW, BF, kSlots, kLSlot3, kCnt3, true>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3,
-                                                          d.ld3, part, gp.w3, kU, 3, Wl);
-    if (threadIdx.x < 3 * TM) db3 += Z3[(threadIdx.x / 3) * d.ld3 + threadIdx.x % 3];
+    if constexpr (FILM) {
+      // the tile's camera (every wave alike) ; wave 0 hands the running
+      // embedding gradient over when the camera changes ; H2 = the layer-3
+      // input from relu(z2) (forward_hidden's layer 2), a thread per column
+      const int nvalid = (int)min((int64_t)TM, M - row0);
+      const int c0 = Cs[0];
+      const int tc = __all(lane >= nvalid || Cs[lane] == c0) ? c0 : -1;  // TM <= 64
+      if (wave == 0) {
+        const int rc = Run[1];
+        if (tc >= 0 && rc >= 0 && rc != tc) {
+          for (int c = lane; c < kU; c += 64) {
+            unsafeAtomicAdd(gembed + (size_t)rc * kU + c, Eacc[c]);
+            Eacc[c] = 0.0f;
+          }
+        }
+        if (lane == 0) {
+          Run[0] = tc;
+          if (tc >= 0) Run[1] = tc;
+        }
+      }
+      static_assert(NT % kU == 0, "a thread keeps one column");
+      const int c = threadIdx.x % kU;
+      const bool film = c >= kU / 2;
+      float esc = 1.0f, esh = 0.0f;
+      if (film && tc >= 0) {
+        esc = w.embed[(size_t)tc * kU + c - kU / 2];
+        esh = w.embed[(size_t)tc * kU + c];
+      }
+      for (int i = threadIdx.x / kU; i < TM; i += NT / kU) {
+        const float h = H2p[i * d.ldU + c];
+        float out = h;
+        if (film) {
+          if (tc >= 0) {
+            out = esc * h + esh;
+          } else {
+            const float* emb = w.embed + (size_t)Cs[i] * kU;
+            out = emb[c - kU / 2] * h + emb[c];
+          }
+        }
+        H2[i * d.ldU + c] = out;
+      }
+    } else {
+      wgrad_phase<TM, NW, BF, kSlots, kLSlot3, kCnt3, true>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3,
+                                                            d.ld3, part, gp.w3, kU, 3, Wl);
+      if (threadIdx.x < 3 * TM) db3 += Z3[(threadIdx.x / 3) * d.ld3 + threadIdx.x % 3];
+    }
     // the dZ2 strip's B chunks (column tile wave), requested before the barrier
     StripPre<BF, 32> pre3;
     strip_preload<32>(op.g3(), op.k3(), 16 * wave, pre3);
@@ -252,22 +332,71 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
       zero_acc(acc);
       mma_strip<RT, 32>(Z3, d.ld3, op.k3(), op.g3(), 16 * ct, acc, pre3);
       const int col = 16 * ct + (lane & 15);
+      if constexpr (FILM) {
+        // dH2 -> the embedding's gradient (tile camera: column sums into
+        // Eacc; several cameras: per row into gembed) and d relu(z2) =
+        // dH2 * scale on the second half; dZ2 over H2p
+        const bool film = 16 * ct >= kU / 2;  // wave-uniform
+        const int tcam = Run[0];
+        const float tscale = (film && tcam >= 0) ? w.embed[(size_t)tcam * kU + col - kU / 2] : 0.0f;
+        float sh = 0.0f, sg = 0.0f;
 #pragma unroll
-      for (int q = 0; q < 4 * RT; ++q) {
-        const int rt = q / 4, e = q % 4;
-        const int row = 16 * rt + 4 * (lane >> 4) + e;
-        const float h = H2[row * d.ldU + col];
-        const float dz = h > 0.0f ? acc[rt][e] : 0.0f;
-        H2[row * d.ldU + col] = dz;
-        db2 += dz;
+        for (int q = 0; q < 4 * RT; ++q) {
+          const int rt = q / 4, e = q % 4;
+          const int row = 16 * rt + 4 * (lane >> 4) + e;
+          float g = acc[rt][e];
+          const float hp = H2p[row * d.ldU + col];
+          if (film) {
+            if (row0 + row < M) {
+              if (tcam >= 0) {
+                sh += g * hp;
+                sg += g;
+              } else {
+                float* gemb = gembed + (size_t)Cs[row] * kU;
+                unsafeAtomicAdd(gemb + col - kU / 2, g * hp);
+                unsafeAtomicAdd(gemb + col, g);
+              }
+            }
+            g = g * (tcam >= 0 ? tscale : w.embed[(size_t)Cs[row] * kU + col - kU / 2]);
+          }
+          const float dz = hp > 0.0f ? g : 0.0f;
+          H2p[row * d.ldU + col] = dz;
+          db2 += dz;
+        }
+        if (film && tcam >= 0) {
+          sh += __shfl_xor(sh, 16, 64);
+          sh += __shfl_xor(sh, 32, 64);
+          sg += __shfl_xor(sg, 16, 64);
+          sg += __shfl_xor(sg, 32, 64);
+          if (lane < 16) {  // the item covers every row of the column: plain adds
+            Eacc[col - kU / 2] += sh;
+            Eacc[col] += sg;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4 * RT; ++q) {
+          const int rt = q / 4, e = q % 4;
+          const int row = 16 * rt + 4 * (lane >> 4) + e;
+          const float h = H2[row * d.ldU + col];
+          const float dz = h > 0.0f ? acc[rt][e] : 0.0f;
+          H2[row * d.ldU + col] = dz;
+          db2 += dz;
+        }
       }
+    }
+    if constexpr (FILM) {
+      // dW3 += H2^T dZ3 ; db3 (H2 stays the layer-3 input through this phase)
+      wgrad_phase<TM, NW, BF, kSlots, kLSlot3, kCnt3, true>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3,
+                                                            d.ld3, part, gp.w3, kU, 3, Wl);
+      if (threadIdx.x < 3 * TM) db3 += Z3[(threadIdx.x / 3) * d.ld3 + threadIdx.x % 3];
     }
     __syncthreads();
     // T3: the next tile's loads (nothing in this phase waits on vmcnt) ;
     // dW2 += H1^T dZ2
-    kept_issue<TM, NT, BF>(d, feats, dirs, act, rgb_in, grgb, row0 + (int64_t)gridDim.x * TM, M,
-                           rpr, nx);
-    wgrad_phase<TM, NW, BF, kSlots, kSlot2, kCnt2>(wacc, wave, wt.n2, kU / 16, H1, d.ldU, H2,
+    kept_issue<TM, NT, BF, FILM>(d, feats, dirs, act, rgb_in, grgb,
+                                 row0 + (int64_t)gridDim.x * TM, M, rpr, nx, cams);
+    wgrad_phase<TM, NW, BF, kSlots, kSlot2, kCnt2>(wacc, wave, wt.n2, kU / 16, H1, d.ldU, DZ2,
                                                    d.ldU, part, gp.w2, kU, kU);
     // the dZ1 strip's first B chunks, requested before the barrier
     StripPre<BF, kU> pre2;
@@ -278,7 +407,7 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
       const int ct = wave;
       f32x4 acc[RT];
       zero_acc(acc);
-      mma_strip<RT, kU>(H2, d.ldU, kU, op.g2(), 16 * ct, acc, pre2);
+      mma_strip<RT, kU>(DZ2, d.ldU, kU, op.g2(), 16 * ct, acc, pre2);
       const int col = 16 * ct + (lane & 15);
 #pragma unroll
       for (int q = 0; q < 4 * RT; ++q) {
@@ -303,7 +432,9 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
         const int c0 = 16 * (it / RT), r0 = 16 * (it % RT);
         f32x4 acc[1];
         zero_acc(acc);
-        if constexpr (BF) {
+        if constexpr (FILM) {
+          mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
+        } else if constexpr (BF) {
           mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, LdsB{G1s, kG1Ld}, c0, acc);
         } else {
           const bool lds = c0 < kG1RowsF;  // wave-uniform
@@ -365,6 +496,13 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
         if (grow < M && col < d.F) gfeat[grow * d.F + col] = acc[0][e];
       }
     }
+  }
+  if constexpr (FILM) {
+    __syncthreads();
+    const int rc = Run[1];
+    if (rc >= 0)
+      for (int c = threadIdx.x; c < kU; c += blockDim.x)
+        unsafeAtomicAdd(gembed + (size_t)rc * kU + c, Eacc[c]);
   }
   // bias sums: db1 / db2 over the lane groups, db3 over the threads of a
   // colour, in X (every wave is past its last tile after the barrier)

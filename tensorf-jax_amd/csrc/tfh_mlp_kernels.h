@@ -515,24 +515,60 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
 }
 
 // layer 3 + sigmoid: R3[row][c] for c < 3 (other columns untouched).
+// BF16: one 16x16x32 MFMA strip per row tile (4 products). FP32: the strip
+// would be 32 dependent 16x16x4 products on two waves while the rest wait at
+// the barrier, so every wave takes part on the vector ALUs instead: 16 lanes
+// per row, lane q sums k = 8q .. 8q+7 for the three colours, then a 16-lane
+// butterfly (mlp_fwd 1.98 -> 1.92 ms at 2.1 M rows; the same on the BF16
+// MLP was 2% slower than its 4 MFMAs).
 template <int TM, int NW, bool BF>
 __device__ void forward_out(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
                             const float* H2, float* R3) {
-  constexpr int RT = TM / 16;
-  const int wave = threadIdx.x >> 6;
-  const int lane = lane_id();
-  for (int rt = wave; rt < RT; rt += NW) {
-    f32x4 acc[1];
-    zero_acc(acc);
-    mma_strip<1, kU>(H2 + 16 * rt * d.ldU, d.ldU, kU, op.l3(), 0, acc);
-    const int col = lane & 15;
-    if (col < 3) {
-      const float bias = w.b3[col];
+  if constexpr (BF) {
+    constexpr int RT = TM / 16;
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    for (int rt = wave; rt < RT; rt += NW) {
+      f32x4 acc[1];
+      zero_acc(acc);
+      mma_strip<1, kU>(H2 + 16 * rt * d.ldU, d.ldU, kU, op.l3(), 0, acc);
+      const int col = lane & 15;
+      if (col < 3) {
+        const float bias = w.b3[col];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = 16 * rt + 4 * (lane >> 4) + e;
-        const float z = acc[0][e] + bias;
-        R3[row * d.ld3 + col] = 1.0f / (1.0f + expf(-z));
+        for (int e = 0; e < 4; ++e) {
+          const int row = 16 * rt + 4 * (lane >> 4) + e;
+          const float z = acc[0][e] + bias;
+          R3[row * d.ld3 + col] = 1.0f / (1.0f + expf(-z));
+        }
+      }
+    }
+  } else {
+    static_assert(kU == 16 * 8, "16 lanes x 8 inputs per row");
+    // TM * 16 is a multiple of 64: a wave's lanes all take a trip or none
+    for (int t = threadIdx.x; t < TM * 16; t += NW * 64) {
+      const int i = t >> 4, q = t & 15;
+      const float4 h0 = *reinterpret_cast<const float4*>(H2 + i * d.ldU + 8 * q);
+      const float4 h1 = *reinterpret_cast<const float4*>(H2 + i * d.ldU + 8 * q + 4);
+      const float h[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+      float a[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float4 w0 = *reinterpret_cast<const float4*>(op.p.ff3 + c * kU + 8 * q);
+        const float4 w1 = *reinterpret_cast<const float4*>(op.p.ff3 + c * kU + 8 * q + 4);
+        const float wv[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+        float s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s += h[j] * wv[j];
+        a[c] = s;
+      }
+#pragma unroll
+      for (int o = 8; o >= 1; o >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[c] += __shfl_xor(a[c], o, 64);
+      if (q < 3) {
+        const float z = (q == 0 ? a[0] : (q == 1 ? a[1] : a[2])) + w.b3[q];
+        R3[i * d.ld3 + q] = 1.0f / (1.0f + expf(-z));
       }
     }
   }
