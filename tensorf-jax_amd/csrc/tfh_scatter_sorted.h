@@ -755,13 +755,20 @@ __global__ void __launch_bounds__(kGatherThreads) k_line_scatter_two(Src src, in
 // adds go straight to the line gradients (no copies).
 static_assert(3 * (kSortMaxGrid + 1) <= kSortMaxTiles, "line bins fit the sort's tile table");
 
-template <class Src>
+// RAY: the pairs in ray order instead, mode by mode (pair e = (row e % M,
+// mode e / M), M rows): a group walks 64 consecutive samples of one ray and
+// mode, and a run is a stretch of the ray inside one line cell; the adds go
+// to fa's / fd's line copies (block b into copy b % nlc), since every ray
+// crosses the same few line texels.
+template <class Src, bool RAY = false>
 __global__ void __launch_bounds__(256) k_line_walk_two(Src src, int64_t pairs,
                                                        const uint32_t* __restrict__ order,
                                                        RowGrad ga, FieldOut fa, ScalarGrad gd,
-                                                       FieldOut fd) {
+                                                       FieldOut fd, int64_t M = 0) {
   constexpr int LPS = 16;
   const int G = (int)fa.vm.grid_dim, CA = (int)fa.vm.channels, CD = (int)fd.vm.channels;
+  float* glA = fa.gline + (int64_t)(blockIdx.x % fa.nlc) * 3 * G * CA;
+  float* glD = fd.gline + (int64_t)(blockIdx.x % fd.nlc) * 3 * G * CD;
   const int l = threadIdx.x & 15;
   const int gbase = threadIdx.x & ~15;
   const int gl0 = gbase & 63;
@@ -781,14 +788,14 @@ __global__ void __launch_bounds__(256) k_line_walk_two(Src src, int64_t pairs,
         const float x0 = quad_to_lane(v0, q, l, gl0), x1 = quad_to_lane(v1, q, l, gl0);
         const int c = 16 * q + l;
         if (c < CA) {
-          if (x0 != 0.0f) unsafeAtomicAdd(fa.gline + (size_t)cell0 * CA + c, x0);
-          if (x1 != 0.0f) unsafeAtomicAdd(fa.gline + (size_t)cell1 * CA + c, x1);
+          if (x0 != 0.0f) unsafeAtomicAdd(glA + (size_t)cell0 * CA + c, x0);
+          if (x1 != 0.0f) unsafeAtomicAdd(glA + (size_t)cell1 * CA + c, x1);
         }
       }
     }
     if (d_on) {
-      if (d0 != 0.0f) unsafeAtomicAdd(fd.gline + (size_t)cell0 * CD + l, d0);
-      if (d1 != 0.0f) unsafeAtomicAdd(fd.gline + (size_t)cell1 * CD + l, d1);
+      if (d0 != 0.0f) unsafeAtomicAdd(glD + (size_t)cell0 * CD + l, d0);
+      if (d1 != 0.0f) unsafeAtomicAdd(glD + (size_t)cell1 * CD + l, d1);
     }
     a0 = a1 = make_float4(0.f, 0.f, 0.f, 0.f);
     d0 = d1 = 0.0f;
@@ -798,7 +805,12 @@ __global__ void __launch_bounds__(256) k_line_walk_two(Src src, int64_t pairs,
     uint32_t pk = 0xffffffffu;
     float px = 0.f, py = 0.f, pz = 0.f;
     if (e < last) {
-      pk = order[e];
+      if constexpr (RAY) {
+        const int64_t m = e / M;
+        pk = (uint32_t)(((e - m * M) << 2) | m);
+      } else {
+        pk = order[e];
+      }
       src.get((int64_t)(pk >> 2), px, py, pz);
     }
     const int nb = (int)min((int64_t)LPS, last - b0);
@@ -888,13 +900,17 @@ inline bool two_field_scatter_ok(const tfh_vm& den, const tfh_vm& app, const Row
 // Both fields' plane and line passes over the pairs sort_pairs ordered.
 // lines_sorted: the line pass on pairs re-sorted by line cell (the sort's
 // arrays are reused after the plane pass), adding straight into lines_a /
-// lines_d (the fields' line gradients); else the ray-order pass into fa's
-// and fd's line copies.
+// lines_d (the fields' line gradients); else a ray-order pass into fa's
+// and fd's line copies: ray_walk, runs of a ray's samples in one line cell
+// summed in registers (k_line_walk_two in ray order; pays where many
+// consecutive samples share a cell), or k_line_scatter_two's merge of each
+// wave's four samples.
 template <class Src>
 inline int32_t scatter_two_fields(const Src& src, int64_t M, const SortWs& s, const RowGrad& ga,
                                   const FieldOut& fa, const ScalarGrad& gd, const FieldOut& fd,
                                   hipStream_t st, bool lines_sorted = false,
-                                  float* lines_a = nullptr, float* lines_d = nullptr) {
+                                  float* lines_a = nullptr, float* lines_d = nullptr,
+                                  bool ray_walk = false) {
   if (M <= 0) return TFH_OK;
   const int64_t pairs = 3 * M;
   const int64_t groups = ceil_div(pairs, kPairsPerGroup);
@@ -917,8 +933,12 @@ inline int32_t scatter_two_fields(const Src& src, int64_t M, const SortWs& s, co
     return check_launch("vm scatter: sorted lines of both fields");
   }
   ProfScope ps("factor_scatter_lines", st);
-  hipLaunchKernelGGL((k_line_scatter_two<Src>), dim3((unsigned)ceil_div(M, kGatherThreads / 16)),
-                     dim3(kGatherThreads), 0, st, src, M, ga, fa, gd, fd);
+  if (ray_walk)
+    hipLaunchKernelGGL((k_line_walk_two<Src, true>), dim3((unsigned)ceil_div(groups, 16)),
+                       dim3(256), 0, st, src, pairs, nullptr, ga, fa, gd, fd, M);
+  else
+    hipLaunchKernelGGL((k_line_scatter_two<Src>), dim3((unsigned)ceil_div(M, kGatherThreads / 16)),
+                       dim3(kGatherThreads), 0, st, src, M, ga, fa, gd, fd);
   return check_launch("vm scatter: lines of both fields");
 }
 
