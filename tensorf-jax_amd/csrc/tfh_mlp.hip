@@ -201,8 +201,10 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
     return v && strcmp(v, "partial") == 0;
   }();
   const bool sd = is_std_dims(d, bf) && !force_partial;
+  const bool kept_film = sd && act && d.ncam > 0;  // k_mlp_bwd_kept<BF, true>
   const size_t lds = bwd_lds_floats<kBwdTM>(d, sd ? kLdsSlots * kBwdNW : 0) * sizeof(float) +
-                     (sd && d.ncam == 0 && kBwdNW == 8 ? (bf ? kG1LdsBytes : kG1LdsBytesF) : 0);
+                     (sd && d.ncam == 0 && kBwdNW == 8 ? (bf ? kG1LdsBytes : kG1LdsBytesF) : 0) +
+                     (kept_film ? (bf ? kG1FilmBytes : kG1FilmBytesF) : 0);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp bwd: tile needs %zu bytes of LDS", lds);
   {
     ProfScope ps("mlp_bwd", st);

@@ -25,8 +25,8 @@
 // T1 forms the layer-3 input H2 = e[:U/2] * h2 + e[U/2:] on the second half
 // of the columns (e = embed[cam]; read once per tile when its rows share a
 // camera), T2 runs dW3 beside the dZ2 strip, whose epilogue also takes the
-// embedding's gradient and leaves dZ2 in H2p, and the dX strip reads its B
-// table from L2 (the LDS copy gives way to H2p).
+// embedding's gradient and leaves dZ2 in H2p, and the LDS copy of the dX
+// strip's B table holds fewer rows (the rest read L2).
 #pragma once
 
 #include "tfh_mlp_kernels.h"
@@ -198,18 +198,19 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
     for (int c = threadIdx.x; c < kU; c += blockDim.x) Eacc[c] = 0.0f;
     if (threadIdx.x == 0) Run[1] = -1;
   }
-  // the first rows of the dX strip's B table in LDS (as k_mlp_bwd; not with FILM)
+  // the first rows of the dX strip's B table in LDS (as k_mlp_bwd; fewer
+  // with FILM)
   __bf16* G1s = reinterpret_cast<__bf16*>(Wl + kLdsSlots * NW * 256);
   float* G1f = Wl + kLdsSlots * NW * 256;
-  if constexpr (FILM) {
-  } else if constexpr (BF) {
-    for (int e = threadIdx.x; e < kG1Rows * (kU / 8); e += blockDim.x) {
+  constexpr int kG1B = FILM ? kG1RowsFilm : kG1Rows, kG1F = FILM ? kG1RowsFilmF : kG1RowsF;
+  if constexpr (BF) {
+    for (int e = threadIdx.x; e < kG1B * (kU / 8); e += blockDim.x) {
       const int j = e / (kU / 8), c8 = e - j * (kU / 8);
       *reinterpret_cast<uint4*>(G1s + j * kG1Ld + 8 * c8) =
           *reinterpret_cast<const uint4*>(pk.g1 + j * kU + 8 * c8);
     }
   } else {
-    for (int e = threadIdx.x; e < kG1RowsF * (kU / 4); e += blockDim.x) {
+    for (int e = threadIdx.x; e < kG1F * (kU / 4); e += blockDim.x) {
       const int j = e / (kU / 4), c4 = e - j * (kU / 4);
       *reinterpret_cast<float4*>(G1f + j * kG1LdF + 4 * c4) =
           *reinterpret_cast<const float4*>(pk.fg1 + j * kU + 4 * c4);
@@ -432,12 +433,15 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
         const int c0 = 16 * (it / RT), r0 = 16 * (it % RT);
         f32x4 acc[1];
         zero_acc(acc);
-        if constexpr (FILM) {
-          mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, op.g1(), c0, acc);
+        if constexpr (FILM && BF) {
+          // LDS rows or L2 rows through one generic pointer
+          const bool lds = c0 < kG1B;  // wave-uniform
+          const LdsB b{lds ? static_cast<const __bf16*>(G1s) : pk.g1, lds ? kG1Ld : kU};
+          mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, b, c0, acc);
         } else if constexpr (BF) {
           mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, LdsB{G1s, kG1Ld}, c0, acc);
         } else {
-          const bool lds = c0 < kG1RowsF;  // wave-uniform
+          const bool lds = c0 < kG1F;  // wave-uniform
           const FlatB32 b{lds ? static_cast<const float*>(G1f) : pk.fg1, lds ? kG1LdF : kU};
           mma_strip<1, kU>(H1 + r0 * d.ldU, d.ldU, kU, b, c0, acc);
         }

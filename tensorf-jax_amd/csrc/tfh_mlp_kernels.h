@@ -490,6 +490,20 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
     mma_strip<RTI, K2>(H1 + r0 * d.ldU, d.ldU, kU, op.l2(), 16 * ct, acc);
     const int col = 16 * ct + (lane & 15);
     const float bias = w.b2[col];
+    // when the tile's rows share a camera (a tile lies in one ray whenever
+    // rays have at least TM rows) its scale and shift of this column are
+    // read once
+    static_assert(TM <= 64, "the camera check reads one tile row per lane");
+    int tc = -1;
+    float esc = 0.0f, esh = 0.0f;
+    if (d.ncam > 0) {
+      const int c0 = Cs[0];
+      tc = __all(lane >= TM || Cs[lane] == c0) ? c0 : -1;
+      if (tc >= 0 && col >= kU / 2) {
+        esc = w.embed[(size_t)tc * kU + col - kU / 2];
+        esh = w.embed[(size_t)tc * kU + col];
+      }
+    }
 #pragma unroll
     for (int rt = 0; rt < RTI; ++rt)
 #pragma unroll
@@ -498,8 +512,12 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
         const float h = fmaxf(acc[rt][e] + bias, 0.0f);
         float out = h;
         if (d.ncam > 0 && col >= kU / 2) {
-          const float* emb = w.embed + (size_t)Cs[row] * kU;
-          out = emb[col - kU / 2] * h + emb[col];
+          if (tc >= 0) {
+            out = esc * h + esh;
+          } else {
+            const float* emb = w.embed + (size_t)Cs[row] * kU;
+            out = emb[col - kU / 2] * h + emb[col];
+          }
         }
         if (H2p != H2) H2p[row * d.ldU + col] = h;
         if (h2g && row < nvalid) {
@@ -847,6 +865,12 @@ constexpr size_t kG1LdsBytes = (size_t)kG1Rows * kG1Ld * 2;
 // 41 KB; 153 KB in all); the other column tiles read L2.
 constexpr int kG1RowsF = 80, kG1LdF = 132;
 constexpr size_t kG1LdsBytesF = (size_t)kG1RowsF * kG1LdF * 4;
+// With the FiLM embedding (the kept-activation backward, whose relu(z2)
+// region takes 16.5 KB of the LDS): the first 80 BF16 rows (21 KB; 158 KB
+// in all) or 48 FP32 rows (25 KB; 154 KB).
+constexpr int kG1RowsFilm = 80, kG1RowsFilmF = 48;
+constexpr size_t kG1FilmBytes = (size_t)kG1RowsFilm * kG1Ld * 2;
+constexpr size_t kG1FilmBytesF = (size_t)kG1RowsFilmF * kG1LdF * 4;
 
 // ACT: H1, H2 (or, with FiLM, relu(z2) before the embedding) and the output
 // come from the forward kernel's act / rgb: only layer 0, the encoding (and
