@@ -213,8 +213,7 @@ int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
     allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV, FILMV>, lds);                         \
     hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, BFV, NTV, STDV, FILMV>),                    \
                        dim3((unsigned)nblk), dim3(kBwdNW * 64), lds, st, d, to_w(p), pk, gp,  \
-                       feats, dirs, cams, grgb, M, rpr, gfeat, partials, g->embed, nullptr,   \
-                       nullptr);                                                              \
+                       feats, dirs, cams, grgb, M, rpr, gfeat, partials, g->embed);          \
   } while (0)
     const bool film = d.ncam > 0;
     if (sd && act && film)

@@ -14,12 +14,12 @@ void launch_mlp_bwd_std(bool bf, unsigned nblk, size_t lds, hipStream_t st, cons
     allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, true, kBwdNT, true, false>, lds);
     hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, true, kBwdNT, true, false>), dim3(nblk),
                        dim3(kBwdNW * 64), lds, st, d, w, pk, gp, feats, dirs, cams, grgb, M, rpr,
-                       gfeat, partials, gembed, nullptr, nullptr);
+                       gfeat, partials, gembed);
   } else {
     allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, false, kBwdNT, true, false>, lds);
     hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, false, kBwdNT, true, false>), dim3(nblk),
                        dim3(kBwdNW * 64), lds, st, d, w, pk, gp, feats, dirs, cams, grgb, M, rpr,
-                       gfeat, partials, gembed, nullptr, nullptr);
+                       gfeat, partials, gembed);
   }
 }
 

@@ -14,12 +14,12 @@ void launch_mlp_bwd_std_film(bool bf, unsigned nblk, size_t lds, hipStream_t st,
     allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, true, kBwdNT, true, true>, lds);
     hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, true, kBwdNT, true, true>), dim3(nblk),
                        dim3(kBwdNW * 64), lds, st, d, w, pk, gp, feats, dirs, cams, grgb, M, rpr,
-                       gfeat, partials, gembed, nullptr, nullptr);
+                       gfeat, partials, gembed);
   } else {
     allow_lds(k_mlp_bwd<kBwdTM, kBwdNW, false, kBwdNT, true, true>, lds);
     hipLaunchKernelGGL((k_mlp_bwd<kBwdTM, kBwdNW, false, kBwdNT, true, true>), dim3(nblk),
                        dim3(kBwdNW * 64), lds, st, d, w, pk, gp, feats, dirs, cams, grgb, M, rpr,
-                       gfeat, partials, gembed, nullptr, nullptr);
+                       gfeat, partials, gembed);
   }
 }
 

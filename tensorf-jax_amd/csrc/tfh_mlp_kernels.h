@@ -1,8 +1,9 @@
-// FeatureMlp kernels on gfx950 MFMA, shared by four translation units:
+// FeatureMlp kernels on gfx950 MFMA, shared by five translation units:
 // tfh_mlp.hip (forward, backward of the generic shapes), tfh_mlp_bwd_std.hip
-// (backward of the standard shape), tfh_mlp_bwd_act.hip (the same reading the
-// activations the forward kept) and tfh_mlp_film.hip (with a FiLM camera
-// embedding). The standard backward sits at the 256-VGPR limit of two waves
+// (backward of the standard shape), tfh_mlp_film.hip (the same with a FiLM
+// camera embedding), tfh_mlp_bwd_act.hip and tfh_mlp_film_act.hip (the
+// backward reading the activations the forward kept, tfh_mlp_bwd_kept.h,
+// without and with the embedding). The standard backward sits at the 256-VGPR limit of two waves
 // per SIMD, and the compiler allocates its registers worse when other
 // instantiations share its translation unit (once 17/31 spilled VGPRs
 // (BF16/FP32) on its own against 26/43 next to the generic kernels and 39/67
@@ -29,7 +30,7 @@
 // [in][out] (input-gradient) tables of the operand precision, L2-resident.
 // The backward recomputes the forward of its tile (only layer 0 and the
 // encoding when the forward kernel kept the hidden activations, the fused
-// training steps' case) and accumulates the weight
+// training steps' case: k_mlp_bwd_kept) and accumulates the weight
 // gradients in registers of their owning wave (standard shape) or in a
 // per-workgroup partial buffer (other shapes, plain read-modify-write, each
 // partial tile owned by one wave), reduced across workgroups by k_mlp_reduce.
@@ -236,26 +237,19 @@ __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
   }
 }
 
-// A tile in one pass, without camera indices: feature rows (Fs, zero padded
-// to FP) and view directions (Ds); with ACT also the activations the forward
-// kernel kept (per row 256 floats: relu(z1), then the layer-3 input; into H1,
-// H2) and dZ3 = grgb * s * (1 - s) from the kept output s (Z3, zero past the
-// 3 colours). Each source is a buffer resource over the tile's valid rows, so
-// an offset is 32 bits and rows past M read 0 with no branch; every load is
-// issued before the first LDS store and the tile pays one memory latency.
-// With CAMS also each row's camera index (Cs, clamped to [0, ncam), from the
-// per-ray cams). Requires F % 4 == 0 and 0 < rpr < 2^31.
-template <int TM, int NT, bool ACT, bool CAMS = false, bool ABF = false>
+// A tile in one pass: feature rows (Fs, zero padded to FP) and view
+// directions (Ds). Each source is a buffer resource over the tile's valid
+// rows, so an offset is 32 bits and rows past M read 0 with no branch; every
+// load is issued before the first LDS store and the tile pays one memory
+// latency. With CAMS also each row's camera index (Cs, clamped to [0, ncam),
+// from the per-ray cams). Requires F % 4 == 0 and 0 < rpr < 2^31.
+template <int TM, int NT, bool CAMS = false>
 __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats,
-                               const float* __restrict__ dirs, const float* __restrict__ act,
-                               const float* __restrict__ rgb, const float* __restrict__ grgb,
-                               int64_t row0, int64_t M, int64_t rpr, float* Fs, float* Ds,
-                               float* H1, float* H2, float* Z3,
+                               const float* __restrict__ dirs, int64_t row0, int64_t M,
+                               int64_t rpr, float* Fs, float* Ds,
                                const uint32_t* __restrict__ cams = nullptr, int* Cs = nullptr) {
   typedef int i32x4 __attribute__((ext_vector_type(4)));
-  // 16-byte loads of kept activations per thread (4 FP32 or 8 BF16 values)
-  constexpr int kA4 = ACT ? TM * (ABF ? 32 : 64) / NT : 1;
-  static_assert(TM * (ABF ? 32 : 64) % NT == 0 && TM * 3 <= NT, "tile shape");
+  static_assert(TM * 3 <= NT, "tile shape");
   const int nv = (int)min((int64_t)TM, M - row0);
   const int64_t ray0 = row0 / rpr;
   const unsigned rem0 = (unsigned)(row0 - ray0 * rpr);
@@ -266,13 +260,6 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
   const float* fbase = feats + row0 * d.F;
   asm volatile("" : "+s"(fbase));
   const auto rf = buffer_rsrc(fbase, nv * d.F * 4);
-  const auto ra = buffer_rsrc(
-      ACT ? static_cast<const void*>(reinterpret_cast<const char*>(act) +
-                                     row0 * kKeptRowBytes<ABF>)
-          : feats,
-      ACT ? nv * kKeptRowBytes<ABF> : 0);
-  const auto rs = buffer_rsrc(ACT ? rgb + row0 * 3 : feats, ACT ? nv * 12 : 0);
-  const auto rg = buffer_rsrc(ACT ? grgb + row0 * 3 : feats, ACT ? nv * 12 : 0);
   const auto rd = buffer_rsrc(dirs + ray0 * 3, nray * 12);
   const auto rc = buffer_rsrc(CAMS ? static_cast<const void*>(cams + ray0) : feats,
                               CAMS ? nray * 4 : 0);
@@ -281,7 +268,6 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
   const int t = threadIdx.x;
   // issue
   i32x4 vf[(TM * 160 / 4 + NT - 1) / NT];  // room for FP <= 160
-  i32x4 va[kA4];
 #pragma unroll
   for (int k = 0; k < (TM * 160 / 4 + NT - 1) / NT; ++k) {
     if (k < kF4) {
@@ -290,16 +276,8 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
       vf[k] = __builtin_amdgcn_raw_buffer_load_b128(rf, off, 0, 0);
     }
   }
-  float s = 0.0f, g = 0.0f;
   const int i3 = t / 3, c3 = t - 3 * i3;
   const bool own3 = t < TM * 3;
-  if constexpr (ACT) {
-#pragma unroll
-    for (int k = 0; k < kA4; ++k)
-      va[k] = __builtin_amdgcn_raw_buffer_load_b128(ra, (t + k * NT) * 16, 0, 0);
-    s = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, own3 ? t * 4 : kOOB, 0, 0));
-    g = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rg, own3 ? t * 4 : kOOB, 0, 0));
-  }
   const unsigned rel = (rem0 + (unsigned)i3) / (unsigned)rpr;  // ray of row i3 after ray0
   const float dv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
       rd, (own3 && i3 < nv) ? (int)(rel * 3 + c3) * 4 : kOOB, 0, 0));
@@ -321,28 +299,6 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
   if (own3) Ds[i3 * 3 + c3] = dv;
   if constexpr (CAMS) {
     if (t < TM) Cs[t] = (cam < 0 || cam >= d.ncam) ? 0 : cam;
-  }
-  if constexpr (ACT) {
-#pragma unroll
-    for (int k = 0; k < kA4; ++k) {
-      if constexpr (ABF) {
-        const int e = t + k * NT, i = e >> 5, c8 = e & 31;
-        float* dst = c8 < 16 ? H1 + i * d.ldU + 8 * c8 : H2 + i * d.ldU + 8 * (c8 - 16);
-        float4 lo, hi;
-        bf16x8_to_f32(__builtin_bit_cast(int4, va[k]), lo, hi);
-        *reinterpret_cast<float4*>(dst) = lo;
-        *reinterpret_cast<float4*>(dst + 4) = hi;
-      } else {
-        const int e = t + k * NT, i = e >> 6, c4 = e & 63;
-        float* dst = c4 < 32 ? H1 + i * d.ldU + 4 * c4 : H2 + i * d.ldU + 4 * (c4 - 32);
-        *reinterpret_cast<float4*>(dst) = __builtin_bit_cast(float4, va[k]);
-      }
-    }
-    if (own3) Z3[i3 * d.ld3 + c3] = g * s * (1.0f - s);
-    for (int e = t; e < TM * (d.Z3W - 3); e += NT) {
-      const int i = e / (d.Z3W - 3), c = 3 + e % (d.Z3W - 3);
-      Z3[i * d.ld3 + c] = 0.0f;
-    }
   }
 }
 
@@ -600,8 +556,9 @@ inline size_t fwd_lds_floats(const MlpDims& d) {
   return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + 4);
 }
 
-// act (optional, no camera embedding): each row's relu(z1) and layer-3 input,
-// 256 floats, kept for the backward (load_tile_rows).
+// act (optional): each row's relu(z1) and layer-3 input (relu(z2) before
+// the camera embedding when there is one), 256 values (FP32, or BF16 with
+// the BF16 MLP), kept for the backward (k_mlp_bwd_kept).
 template <int TM, int NW, bool BF, bool STD>
 __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpPack pk,
                                                      const float* __restrict__ feats,
@@ -623,11 +580,9 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   const Operands<BF> op{d, w, pk};
   const int64_t row0 = (int64_t)blockIdx.x * TM;
   if (STD && d.ncam == 0)
-    load_tile_rows<TM, NW * 64, false>(d, feats, dirs, nullptr, nullptr, nullptr, row0, M, rpr,
-                                       regA, Ds, nullptr, nullptr, nullptr, nullptr, nullptr);
+    load_tile_rows<TM, NW * 64>(d, feats, dirs, row0, M, rpr, regA, Ds);
   else if (STD)
-    load_tile_rows<TM, NW * 64, false, true>(d, feats, dirs, nullptr, nullptr, nullptr, row0, M,
-                                             rpr, regA, Ds, nullptr, nullptr, nullptr, cams, Cs);
+    load_tile_rows<TM, NW * 64, true>(d, feats, dirs, row0, M, rpr, regA, Ds, cams, Cs);
   else
     load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, regA, Ds, Cs);
   __syncthreads();
@@ -872,10 +827,10 @@ constexpr int kG1RowsFilm = 80, kG1RowsFilmF = 48;
 constexpr size_t kG1FilmBytes = (size_t)kG1RowsFilm * kG1Ld * 2;
 constexpr size_t kG1FilmBytesF = (size_t)kG1RowsFilmF * kG1LdF * 4;
 
-// ACT: H1, H2 (or, with FiLM, relu(z2) before the embedding) and the output
-// come from the forward kernel's act / rgb: only layer 0, the encoding (and
-// the embedding's scale and shift) are recomputed.
-template <int TM, int NW, bool BF, int NT, bool STD, bool FILM, bool ACT = false>
+// The backward that recomputes each tile's forward: callers without the
+// activations kept by the forward kernel (the training steps' backward,
+// which reads them, is k_mlp_bwd_kept in tfh_mlp_bwd_kept.h).
+template <int TM, int NW, bool BF, int NT, bool STD, bool FILM>
 __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpPack pk, GradPack gp_arg,
                                                      const float* __restrict__ feats,
                                                      const float* __restrict__ dirs,
@@ -883,9 +838,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
                                                      const float* __restrict__ grgb, int64_t M,
                                                      int64_t rpr, float* __restrict__ gfeat,
                                                      float* __restrict__ partials,
-                                                     float* __restrict__ gembed,
-                                                     const float* __restrict__ act,
-                                                     const float* __restrict__ rgb_in) {
+                                                     float* __restrict__ gembed) {
   constexpr int RT = TM / 16;
   static_assert(TM <= 64, "the FiLM camera check reads one tile row per lane");
   // row tiles per work item of the strip phases: all of them (the B strip is
@@ -966,13 +919,7 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t row0 = tile * TM;
     __syncthreads();
-    if constexpr (ACT) {
-      // with an embedding the kept second half is relu(z2) before it (H2p)
-      load_tile_rows<TM, NW * 64, true, FILM, BF>(d, feats, dirs, act, rgb_in, grgb, row0, M, rpr, Fs,
-                                              Ds, H1, FILM ? H2p : H2, Z3, cams, Cs);
-    } else {
-      load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
-    }
+    load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, Fs, Ds, Cs);
     __syncthreads();
     if (FILM && d.ncam > 0 && wave == 0) {
       // read before the next use: forward_hidden synchronises the workgroup
@@ -991,52 +938,20 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_bwd(MlpDims d_arg, MlpW w, MlpP
         if (tc >= 0) Run[1] = tc;
       }
     }
-    forward_hidden<TM, NW, BF, BRTI, !ACT>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
-    if constexpr (!ACT) forward_out<TM, NW, BF>(d, w, op, H2, Z3);
-    if constexpr (ACT && FILM) {
-      // the layer-3 input from the kept relu(z2): the embedding's scale and
-      // shift on the second half of the columns (forward_hidden's layer 2).
-      // A thread keeps one column (blockDim.x is a multiple of kU); when the
-      // tile's rows share a camera (a tile lies in one ray whenever rays
-      // have at least TM rows) its scale and shift are read once
-      static_assert((NW * 64) % kU == 0, "a thread keeps one column");
-      const int c = threadIdx.x % kU;
-      const int tcam = d.ncam > 0 ? Run[0] : -1;
-      float esc = 0.0f, esh = 0.0f;
-      if (d.ncam > 0 && c >= kU / 2 && tcam >= 0) {
-        esc = w.embed[(size_t)tcam * kU + c - kU / 2];
-        esh = w.embed[(size_t)tcam * kU + c];
-      }
-      for (int i = threadIdx.x / kU; i < TM; i += blockDim.x / kU) {
-        const float h = H2p[i * d.ldU + c];
-        float out = h;
-        if (d.ncam > 0 && c >= kU / 2) {
-          if (tcam >= 0) {
-            out = esc * h + esh;
-          } else {
-            const float* emb = w.embed + (size_t)Cs[i] * kU;
-            out = emb[c - kU / 2] * h + emb[c];
-          }
-        }
-        H2[i * d.ldU + c] = out;
-      }
-      __syncthreads();
-    }
+    forward_hidden<TM, NW, BF, BRTI>(d, w, op, Fs, X, H0, H1, H2p, H2, Ds, Cs);
+    forward_out<TM, NW, BF>(d, w, op, H2, Z3);
     // dZ3 = grgb * s * (1 - s), zero outside the 3 colours and the valid rows
-    // (loaded with the tile when the activations are kept)
-    if constexpr (!ACT) {
-      for (int e = threadIdx.x; e < TM * d.Z3W; e += blockDim.x) {
-        const int i = e / d.Z3W, c = e % d.Z3W;
-        const int64_t row = row0 + i;
-        float v = 0.0f;
-        if (c < 3 && row < M) {
-          const float s = Z3[i * d.ld3 + c];
-          v = grgb[row * 3 + c] * s * (1.0f - s);
-        }
-        Z3[i * d.ld3 + c] = v;
+    for (int e = threadIdx.x; e < TM * d.Z3W; e += blockDim.x) {
+      const int i = e / d.Z3W, c = e % d.Z3W;
+      const int64_t row = row0 + i;
+      float v = 0.0f;
+      if (c < 3 && row < M) {
+        const float s = Z3[i * d.ld3 + c];
+        v = grgb[row * 3 + c] * s * (1.0f - s);
       }
-      __syncthreads();
+      Z3[i * d.ld3 + c] = v;
     }
+    __syncthreads();
     // dW3 (U x 3) += H2^T dZ3 ; db3
     wgrad_phase<TM, NW, BF, NT, kLSlot3, kCnt3, true>(wacc, wave, wt.n3, 1, H2, d.ldU, Z3, d.ld3,
                                                       part, gp.w3, kU, 3, Wl);
