@@ -477,18 +477,22 @@ __global__ void __launch_bounds__(kKeptNW * 64) k_mlp_bwd_kept(
         H0[i * d.ldS + j] = g;
       }
     }
-    // the first dF item's B chunks (item wave: column tile wave / RT)
+    // the first dF item's B chunks. The dF items count down from the last
+    // wave while the dW0 tiles count up from wave 0, so a wave with a third
+    // dW0 tile has two dF items (FP32: 9 tile products per SIMD instead of
+    // 10 on two of them)
+    const int it0 = NW - 1 - wave;
     StripPre<BF, kSP> pre0;
-    strip_preload<kSP>(op.g0(), kSP, 16 * (wave / RT), pre0);
+    strip_preload<kSP>(op.g0(), kSP, 16 * (it0 / RT), pre0);
     __syncthreads();
     // T7: dW0 += Fs^T dH0 ; dF = dH0 @ W0^T -> gfeat
     wgrad_phase<TM, NW, BF, kSlots, kLSlot0, kCnt0, true>(wacc, wave, wt.n0, 2, Fs, d.ldF, H0,
                                                           d.ldS, part, gp.w0, d.F, d.S, Wl);
-    for (int it = wave; it < (d.FP / 16) * RT; it += NW) {
+    for (int it = it0; it < (d.FP / 16) * RT; it += NW) {
       const int ct = it / RT, r0 = 16 * (it % RT);
       f32x4 acc[1];
       zero_acc(acc);
-      if (it == wave)
+      if (it == it0)
         mma_strip<1, kSP>(H0 + r0 * d.ldS, d.ldS, kSP, op.g0(), 16 * ct, acc, pre0);
       else
         mma_strip<1, kSP>(H0 + r0 * d.ldS, d.ldS, kSP, op.g0(), 16 * ct, acc);
