@@ -675,7 +675,8 @@ static int32_t render_forward(const tfh_vm* density, const tfh_vm* appearance, c
   RaySamples src{ws.d.pts, ws.d.idx, k, N};
   {
     ProfScope ps("appearance_gather", st);
-    TFH_TRY((launch_vm_gather<RaySamples, true>(*appearance, src, M, ws.feats, st)));
+    TFH_TRY((launch_vm_gather<RaySamples, true>(*appearance, src, M, ws.feats, st,
+                                                 k >= (int)appearance->grid_dim)));
   }
   {
     ProfScope ps("mlp_fwd", st);

@@ -105,16 +105,20 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_gather(tfh_vm vm, Src src
 
 // Row-major gather with 16-byte taps: LPS lanes per sample, lane l owns
 // channels 4l .. 4l+3 of every mode (LPS = C/4 rounded up to a power of
-// two); a quarter of the load instructions of k_vm_gather, the same values.
+// two, or 12 for 48 channels: five groups per wave and 4 lanes idle, where
+// 16 lanes per sample leave a quarter of them idle, and every lane repeats
+// its sample's tap and address work); a quarter of the load instructions of
+// k_vm_gather, the same values.
 template <class Src, int LPS>
 __global__ void __launch_bounds__(kGatherThreads) k_vm_gather4(tfh_vm vm, Src src, int64_t M,
                                                                float* __restrict__ out) {
-  constexpr int SPB = kGatherThreads / LPS;
-  const int grp = threadIdx.x / LPS, l = threadIdx.x % LPS;
+  constexpr int GPW = 64 / LPS, SPB = (kGatherThreads / 64) * GPW;  // groups per wave, block
+  const int lane = threadIdx.x & 63;
+  const int grp = (threadIdx.x >> 6) * GPW + lane / LPS, l = lane % LPS;
   const int64_t s = (int64_t)blockIdx.x * SPB + grp;
   const int G = (int)vm.grid_dim, C = (int)vm.channels;
   const int c4 = 4 * l;
-  if (s >= M || c4 >= C) return;  // no barrier or lane exchange below
+  if (lane >= GPW * LPS || s >= M || c4 >= C) return;  // no barrier or lane exchange below
   float x = 0.f, y = 0.f, z = 0.f;
   src.get(s, x, y, z);
   ModeTaps mt[3];
@@ -132,14 +136,19 @@ __global__ void __launch_bounds__(kGatherThreads) k_vm_gather4(tfh_vm vm, Src sr
 }
 
 template <class Src, bool ROW_MAJOR>
+// dense: the rows are samples along rays with more samples than the grid has
+// cells, whose neighbours in a wave share most texels; 48 channels then take
+// 12 lanes per sample (uniform-512 gather 0.55 -> 0.47 ms), while sparser
+// rows keep 16 (proposal-64: 0.40 against 0.42 ms with 12, the five samples
+// of a wave touching more distinct lines per load).
 inline int32_t launch_vm_gather(const tfh_vm& vm, const Src& src, int64_t M, float* out,
-                                hipStream_t st) {
+                                hipStream_t st, bool dense = false) {
   const int C = (int)vm.channels;
   if (ROW_MAJOR && C % 4 == 0 && C <= 64 && (reinterpret_cast<uintptr_t>(out) & 15) == 0 &&
       (reinterpret_cast<uintptr_t>(vm.plane) & 15) == 0 &&
       (reinterpret_cast<uintptr_t>(vm.line) & 15) == 0) {
-    const int lps = C <= 8 ? 2 : (C <= 16 ? 4 : (C <= 32 ? 8 : 16));
-    const int64_t blocks = ceil_div(M, kGatherThreads / lps);
+    const int lps = C <= 8 ? 2 : (C <= 16 ? 4 : (C <= 32 ? 8 : (C == 48 && dense ? 12 : 16)));
+    const int64_t blocks = ceil_div(M, (kGatherThreads / 64) * (64 / lps));
     TFH_REQUIRE(blocks < (1ll << 31), "vm gather: too many points (%lld)", (long long)M);
 #define TFH_G4(L)                                                                            \
   hipLaunchKernelGGL((k_vm_gather4<Src, L>), dim3((unsigned)blocks), dim3(kGatherThreads), 0, \
@@ -147,6 +156,7 @@ inline int32_t launch_vm_gather(const tfh_vm& vm, const Src& src, int64_t M, flo
     if (lps == 2) TFH_G4(2);
     else if (lps == 4) TFH_G4(4);
     else if (lps == 8) TFH_G4(8);
+    else if (lps == 12) TFH_G4(12);
     else TFH_G4(16);
 #undef TFH_G4
     return check_launch("vm gather");
