@@ -257,8 +257,17 @@ struct PreB16 {
   bf16x8 bb[NS];
 };
 
-template <int NR, int KMAX, class B16, bool PRE = false>
-__device__ __forceinline__ void mma_strip_b16(const float* __restrict__ A, int lda, int K, B16 b,
+// A lane's 8 A operands of a BF16 strip: FP32 rows rounded per use, or rows
+// kept in BF16 (one 16-byte read).
+__device__ __forceinline__ bf16x8 strip_a8(const float* ap) {
+  return to_bf16x8(*reinterpret_cast<const float4*>(ap), *reinterpret_cast<const float4*>(ap + 4));
+}
+__device__ __forceinline__ bf16x8 strip_a8(const __bf16* ap) {
+  return *reinterpret_cast<const bf16x8*>(ap);
+}
+
+template <int NR, int KMAX, class B16, bool PRE = false, class AT = float>
+__device__ __forceinline__ void mma_strip_b16(const AT* __restrict__ A, int lda, int K, B16 b,
                                               int col0, f32x4 (&acc)[NR],
                                               const PreB16<KMAX>* pre = nullptr) {
   const int lane = lane_here();
@@ -279,9 +288,7 @@ __device__ __forceinline__ void mma_strip_b16(const float* __restrict__ A, int l
       if (kc + 32 * i < K) {
 #pragma unroll
         for (int rt = 0; rt < NR; ++rt) {
-          const float* ap = A + (16 * rt + r) * lda + kc + 32 * i + 8 * kq;
-          const bf16x8 av = to_bf16x8(*reinterpret_cast<const float4*>(ap),
-                                      *reinterpret_cast<const float4*>(ap + 4));
+          const bf16x8 av = strip_a8(A + (16 * rt + r) * lda + kc + 32 * i + 8 * kq);
           acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bb[i], acc[rt], 0, 0, 0);
         }
       }
@@ -293,6 +300,13 @@ template <int NR, int KMAX>
 __device__ __forceinline__ void mma_strip(const float* __restrict__ A, int lda, int K, PackedB b,
                                           int col0, f32x4 (&acc)[NR]) {
   mma_strip_b16<NR, KMAX>(A, lda, K, b, col0, acc);
+}
+
+// BF16 A rows (lda in BF16 values, 16-byte aligned rows).
+template <int NR, int KMAX>
+__device__ __forceinline__ void mma_strip(const __bf16* __restrict__ A, int lda, int K, PackedB b,
+                                          int col0, f32x4 (&acc)[NR]) {
+  mma_strip_b16<NR, KMAX, PackedB, false, __bf16>(A, lda, K, b, col0, acc);
 }
 
 template <int KMAX>

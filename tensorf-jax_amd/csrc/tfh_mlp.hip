@@ -144,11 +144,12 @@ int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   Carve cv(ws, ws_bytes);
   MlpPack pk;
   TFH_TRY(build_pack(p, d, bf, cv, pk, st));
-  const size_t lds = fwd_lds_floats<kFwdTM>(d) * sizeof(float);
+  const bool sd = is_std_dims(d, bf);
+  // (the BF16 standard shape keeps its feature rows in BF16: k_mlp_fwd)
+  const size_t lds = fwd_lds_floats<kFwdTM>(d, bf && sd) * sizeof(float);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp fwd: tile needs %zu bytes of LDS", lds);
   const int64_t blocks = ceil_div(M, kFwdTM);
   ProfScope ps("mlp_fwd_kernel", st);
-  const bool sd = is_std_dims(d, bf);
 #define TFH_LAUNCH_FWD(BFV, STDV)                                                             \
   do {                                                                                       \
     allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, BFV, STDV>, lds);                                    \

@@ -237,16 +237,22 @@ __device__ void load_tile(const MlpDims& d, const float* __restrict__ feats,
   }
 }
 
+// Row stride (BF16 values) of the BF16 standard forward's feature rows in
+// LDS: 84 dwords, 4 mod 8, so a 16-byte read per lane is conflict free.
+constexpr int kFsLdB = 168;
+
 // A tile in one pass: feature rows (Fs, zero padded to FP) and view
 // directions (Ds). Each source is a buffer resource over the tile's valid
 // rows, so an offset is 32 bits and rows past M read 0 with no branch; every
 // load is issued before the first LDS store and the tile pays one memory
 // latency. With CAMS also each row's camera index (Cs, clamped to [0, ncam),
-// from the per-ray cams). Requires F % 4 == 0 and 0 < rpr < 2^31.
-template <int TM, int NT, bool CAMS = false>
+// from the per-ray cams). FB16: the feature rows go to LDS in BF16 (Fs is
+// then __bf16 with stride kFsLdB), rounded as the BF16 strips round their A
+// operands. Requires F % 4 == 0 and 0 < rpr < 2^31.
+template <int TM, int NT, bool CAMS = false, bool FB16 = false>
 __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats,
                                const float* __restrict__ dirs, int64_t row0, int64_t M,
-                               int64_t rpr, float* Fs, float* Ds,
+                               int64_t rpr, void* Fs, float* Ds,
                                const uint32_t* __restrict__ cams = nullptr, int* Cs = nullptr) {
   typedef int i32x4 __attribute__((ext_vector_type(4)));
   static_assert(TM * 3 <= NT, "tile shape");
@@ -293,8 +299,16 @@ __device__ void load_tile_rows(const MlpDims& d, const float* __restrict__ feats
 #pragma unroll
   for (int k = 0; k < (TM * 160 / 4 + NT - 1) / NT; ++k) {
     const int e = t + k * NT, i = e / fp4, c4 = e - i * fp4;
-    if (k < kF4 && e < n4)
-      *reinterpret_cast<float4*>(Fs + i * d.ldF + 4 * c4) = __builtin_bit_cast(float4, vf[k]);
+    if (k < kF4 && e < n4) {
+      const float4 v = __builtin_bit_cast(float4, vf[k]);
+      if constexpr (FB16) {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        const bf16x4 b = {(__bf16)v.x, (__bf16)v.y, (__bf16)v.z, (__bf16)v.w};
+        *reinterpret_cast<bf16x4*>(static_cast<__bf16*>(Fs) + i * kFsLdB + 4 * c4) = b;
+      } else {
+        *reinterpret_cast<float4*>(static_cast<float*>(Fs) + i * d.ldF + 4 * c4) = v;
+      }
+    }
   }
   if (own3) Ds[i3 * 3 + c3] = dv;
   if constexpr (CAMS) {
@@ -387,14 +401,18 @@ __device__ void encode_tile(const MlpDims& d, float* X, const float* Ds,
 // column strip's B operands for all rows of the tile (few waves), RTI = 1
 // spreads a layer over more waves.
 // L12 == false stops after the encoding (H1 and H2 come from elsewhere).
-template <int TM, int NW, bool BF, int RTI, bool L12 = true>
+// FT: the feature rows' type (float, stride ldF; or __bf16 with the BF16
+// MLP, stride kFsLdB).
+template <int TM, int NW, bool BF, int RTI, bool L12 = true, class FT = float>
 // h2g (optional, rows < nvalid): relu(z2) also stored to global memory, rows
 // of 256 kept values (FP32, or BF16 with the BF16 MLP: the forward's kept
 // activations with an embedding).
 __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<BF>& op,
-                               const float* Fs, float* X, float* H0, float* H1, float* H2p,
+                               const FT* Fs, float* X, float* H0, float* H1, float* H2p,
                                float* H2, const float* Ds, const int* Cs,
                                void* __restrict__ h2g = nullptr, int nvalid = 0) {
+  static_assert(std::is_same<FT, float>::value || BF, "BF16 feature rows need the BF16 MLP");
+  const int ldFs = std::is_same<FT, float>::value ? d.ldF : kFsLdB;
   constexpr int RG = TM / 16 / RTI;  // row groups
   // B prefetch depth (K per pass): the whole strip with few waves, 64 when the
   // workgroup's waves share the register file (16 waves)
@@ -407,7 +425,7 @@ __device__ void forward_hidden(const MlpDims& d, const MlpW& w, const Operands<B
     const int ct = it / (TM / 16), r0 = 16 * (it % (TM / 16));
     f32x4 acc[1];
     zero_acc(acc);
-    mma_strip<1, K0>(Fs + r0 * d.ldF, d.ldF, op.k0(), op.l0(), 16 * ct, acc);
+    mma_strip<1, K0>(Fs + r0 * ldFs, ldFs, op.k0(), op.l0(), 16 * ct, acc);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = r0 + 4 * (lane >> 4) + e, col = 16 * ct + (lane & 15);
@@ -551,9 +569,11 @@ __device__ void forward_out(const MlpDims& d, const MlpW& w, const Operands<BF>&
 
 // ---- forward kernel -----------------------------------------------------------
 template <int TM>
-inline size_t fwd_lds_floats(const MlpDims& d) {
-  // the layer-3 output reuses region A (H1 is dead by then)
-  return (size_t)TM * (std::max(d.ldF, d.ldU) + std::max(d.ldE, d.ldU) + 4 + 4);
+inline size_t fwd_lds_floats(const MlpDims& d, bool fs_b16 = false) {
+  // the layer-3 output reuses region A (H1 is dead by then); BF16 feature
+  // rows take half of it
+  return (size_t)TM * (std::max(fs_b16 ? kFsLdB / 2 : d.ldF, d.ldU) + std::max(d.ldE, d.ldU) +
+                       4 + 4);
 }
 
 // act (optional): each row's relu(z1) and layer-3 input (relu(z2) before
@@ -572,17 +592,21 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   // regions: A = Fs, later H1, last R3 (the layer-3 output); B = X, later
   // H2; small: Ds, Cs. 40 KB per workgroup for the FP32 standard shape: four
   // workgroups share a CU.
+  // the BF16 standard shape keeps its feature rows in BF16 (layer 0 reads
+  // them as BF16 operands either way): 39 KB of LDS, four workgroups per CU
+  // instead of three
+  constexpr bool FB16 = BF && STD;
   float* regA = smem;
-  float* regB = regA + TM * std::max(d.ldF, d.ldU);
+  float* regB = regA + TM * std::max(FB16 ? kFsLdB / 2 : d.ldF, d.ldU);
   float* Ds = regB + TM * std::max(d.ldE, d.ldU);
   int* Cs = reinterpret_cast<int*>(Ds + TM * 4);
   float* R3 = regA;
   const Operands<BF> op{d, w, pk};
   const int64_t row0 = (int64_t)blockIdx.x * TM;
   if (STD && d.ncam == 0)
-    load_tile_rows<TM, NW * 64>(d, feats, dirs, row0, M, rpr, regA, Ds);
+    load_tile_rows<TM, NW * 64, false, FB16>(d, feats, dirs, row0, M, rpr, regA, Ds);
   else if (STD)
-    load_tile_rows<TM, NW * 64, true>(d, feats, dirs, row0, M, rpr, regA, Ds, cams, Cs);
+    load_tile_rows<TM, NW * 64, true, FB16>(d, feats, dirs, row0, M, rpr, regA, Ds, cams, Cs);
   else
     load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, regA, Ds, Cs);
   __syncthreads();
@@ -594,7 +618,9 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   const bool film_keep = act && d.ncam > 0;
   char* kept = reinterpret_cast<char*>(act);
   constexpr int kRow = kKeptRowBytes<BF>;
-  forward_hidden<TM, NW, BF, TM / 16>(d, w, op, regA, regB, nullptr, regA, regB, regB, Ds, Cs,
+  using FT = typename std::conditional<FB16, __bf16, float>::type;
+  forward_hidden<TM, NW, BF, TM / 16, true, FT>(d, w, op, reinterpret_cast<const FT*>(regA), regB,
+                                                nullptr, regA, regB, regB, Ds, Cs,
                                       film_keep ? kept + row0 * kRow + kRow / 2 : nullptr,
                                       (int)min((int64_t)TM, M - row0));
   if (act) {
