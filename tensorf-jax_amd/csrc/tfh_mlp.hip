@@ -1,6 +1,7 @@
 // FeatureMlp (networks.py:38-121) forward and backward: weight packing, the
 // launches and the C entry points. The kernels are in tfh_mlp_kernels.h.
 #include "tfh_mlp_kernels.h"
+#include "tfh_vm_kernels.h"
 
 namespace tfh {
 
@@ -155,7 +156,7 @@ int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
     allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, BFV, STDV>, lds);                                    \
     hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW, BFV, STDV>), dim3((unsigned)blocks),       \
                        dim3(kFwdNW * 64), lds, st, d, to_w(p), pk, feats, dirs, cams, M, rpr,  \
-                       rgb, act);                                                                 \
+                       rgb, act, NoGather{});                                                     \
   } while (0)
   if (bf) {
     if (sd) TFH_LAUNCH_FWD(true, true); else TFH_LAUNCH_FWD(true, false);
@@ -164,6 +165,115 @@ int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, co
   }
 #undef TFH_LAUNCH_FWD
   return check_launch("mlp fwd");
+}
+
+// The appearance gather (TensorVM.interpolate, tensor_vm.py:42-89; the
+// arithmetic of k_vm_gather4, so the rows are the same values) as the tile
+// source of the standard forward: a 16-lane group per row, lane l channels
+// 4l .. 4l+3 of each mode (48 channels: lanes 12..15 write the BF16 rows'
+// zero padding). The rows also go to feats (M, 144) for the backward. The
+// four workgroups of a CU overlap one tile's tap latency with the others'
+// products.
+struct GatherTile {
+  tfh_vm vm;         // appearance field, 48 channels, 16-byte aligned
+  RaySamples src;    // the sample points of the rows
+  float* feats;      // (M, 144) out
+  template <int TM, int NT, bool CAMS, bool FB16>
+  __device__ void load(const MlpDims& d, int64_t row0, int64_t M, int64_t rpr, float* Fs,
+                       float* Ds, int* Cs, const float* __restrict__ dirs,
+                       const uint32_t* __restrict__ cams) const {
+    constexpr int C = 48;
+    static_assert(TM * 16 == NT, "a 16-lane group per row");
+    const int t = threadIdx.x, i = t >> 4, l = t & 15, c4 = 4 * l;
+    const int G = (int)vm.grid_dim;
+    const int64_t row = row0 + i;
+    const bool valid = row < M;
+    float x = 0.f, y = 0.f, z = 0.f;
+    if (valid) src.get(row, x, y, z);
+    ModeTaps mt[3];
+    mode_taps(x, y, z, G, mt);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (valid && c4 < C) {
+        TapAddr a;
+        tap_addrs(mt[m], m, G, a);
+        float4 lv, pv;
+        tap_values4(vm.plane, vm.line, C, c4, a, lv, pv);
+        // tensor_vm.py:167
+        f = make_float4(lv.x * pv.x, lv.y * pv.y, lv.z * pv.z, lv.w * pv.w);
+        *reinterpret_cast<float4*>(feats + row * 3 * C + m * C + c4) = f;
+      }
+      if (c4 < C) {
+        if constexpr (FB16) {
+          typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+          const bf16x4 b = {(__bf16)f.x, (__bf16)f.y, (__bf16)f.z, (__bf16)f.w};
+          *reinterpret_cast<bf16x4*>(reinterpret_cast<__bf16*>(Fs) + i * kFsLdB + m * C + c4) = b;
+        } else {
+          *reinterpret_cast<float4*>(Fs + i * d.ldF + m * C + c4) = f;
+        }
+      }
+    }
+    if constexpr (FB16) {
+      // columns 3C .. FP - 1 (the K padding of layer 0) are zero
+      static_assert(3 * C + 16 == std_dims<true>().FP, "BF16 rows pad 144 to 160");
+      if (l >= 12) {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        const bf16x4 zero = {(__bf16)0.f, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f};
+        *reinterpret_cast<bf16x4*>(reinterpret_cast<__bf16*>(Fs) + i * kFsLdB + 3 * C +
+                                   4 * (l - 12)) = zero;
+      }
+    }
+    if (t < TM * 3) {
+      const int i3 = t / 3, c3 = t - 3 * i3;
+      const int64_t r = row0 + i3;
+      Ds[t] = r < M ? dirs[(r / rpr) * 3 + c3] : 0.0f;
+    }
+    if constexpr (CAMS) {
+      if (t < TM) {
+        const int64_t r = row0 + t;
+        const int cam = r < M ? (int)cams[r / rpr] : 0;
+        Cs[t] = (cam < 0 || cam >= d.ncam) ? 0 : cam;
+      }
+    }
+  }
+};
+
+// The standard forward with the appearance gather as its tile source
+// (GatherTile); *done = false when the shape, channels or alignment do not
+// allow it (the caller then gathers and calls mlp_fwd_rows).
+int32_t mlp_fwd_rows_gather(const tfh_mlp* p, const tfh_vm& vm, const RaySamples& src,
+                            float* feats, const float* dirs, const uint32_t* cams, int64_t M,
+                            int64_t rpr, float* rgb, int32_t precision, void* ws,
+                            size_t ws_bytes, hipStream_t st, float* act, bool* done) {
+  *done = false;
+  const bool bf = precision == TFH_PREC_BF16;
+  MlpDims d = make_dims(p, bf);
+  if (!is_std_dims(d, bf) || vm.channels != 48 || 3 * vm.channels != d.F ||
+      (reinterpret_cast<uintptr_t>(vm.plane) & 15) || (reinterpret_cast<uintptr_t>(vm.line) & 15) ||
+      (reinterpret_cast<uintptr_t>(feats) & 15) || M <= 0)
+    return TFH_OK;
+  Carve cv(ws, ws_bytes);
+  MlpPack pk;
+  TFH_TRY(build_pack(p, d, bf, cv, pk, st));
+  const size_t lds = fwd_lds_floats<kFwdTM>(d, bf) * sizeof(float);
+  TFH_REQUIRE(lds <= 160 * 1024, "mlp fwd: tile needs %zu bytes of LDS", lds);
+  const int64_t blocks = ceil_div(M, kFwdTM);
+  const GatherTile gt{vm, src, feats};
+  ProfScope ps("mlp_fwd_kernel", st);
+  if (bf) {
+    allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, true, true, GatherTile>, lds);
+    hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW, true, true, GatherTile>), dim3((unsigned)blocks),
+                       dim3(kFwdNW * 64), lds, st, d, to_w(p), pk, nullptr, dirs, cams, M, rpr,
+                       rgb, act, gt);
+  } else {
+    allow_lds(k_mlp_fwd<kFwdTM, kFwdNW, false, true, GatherTile>, lds);
+    hipLaunchKernelGGL((k_mlp_fwd<kFwdTM, kFwdNW, false, true, GatherTile>),
+                       dim3((unsigned)blocks), dim3(kFwdNW * 64), lds, st, d, to_w(p), pk,
+                       nullptr, dirs, cams, M, rpr, rgb, act, gt);
+  }
+  *done = true;
+  return check_launch("mlp fwd with the gather");
 }
 
 static int64_t bwd_blocks(int64_t M) {

@@ -579,14 +579,23 @@ inline size_t fwd_lds_floats(const MlpDims& d, bool fs_b16 = false) {
 // act (optional): each row's relu(z1) and layer-3 input (relu(z2) before
 // the camera embedding when there is one), 256 values (FP32, or BF16 with
 // the BF16 MLP), kept for the backward (k_mlp_bwd_kept).
-template <int TM, int NW, bool BF, bool STD>
+// No tile source: the feature rows are read from feats.
+struct NoGather {};
+
+// GT (standard shape only): a tile source that forms the feature rows
+// itself (gt.load<TM, NT, CAMS, FB16>(d, row0, M, rpr, Fs, Ds, Cs, dirs,
+// cams): the appearance gather fused into the tile load, its rows also
+// written to global memory for the backward), instead of reading feats.
+template <int TM, int NW, bool BF, bool STD, class GT = NoGather>
 __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpPack pk,
                                                      const float* __restrict__ feats,
                                                      const float* __restrict__ dirs,
                                                      const uint32_t* __restrict__ cams,
                                                      int64_t M, int64_t rpr,
                                                      float* __restrict__ rgb,
-                                                     float* __restrict__ act) {
+                                                     float* __restrict__ act, GT gt) {
+  constexpr bool kGather = !std::is_same<GT, NoGather>::value;
+  static_assert(!kGather || STD, "the fused gather serves the standard shape");
   const MlpDims d = STD ? std_dims<BF>(d_arg.ncam) : d_arg;
   extern __shared__ float smem[];
   // regions: A = Fs, later H1, last R3 (the layer-3 output); B = X, later
@@ -603,12 +612,18 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   float* R3 = regA;
   const Operands<BF> op{d, w, pk};
   const int64_t row0 = (int64_t)blockIdx.x * TM;
-  if (STD && d.ncam == 0)
+  if constexpr (kGather) {
+    if (d.ncam == 0)
+      gt.template load<TM, NW * 64, false, FB16>(d, row0, M, rpr, regA, Ds, Cs, dirs, cams);
+    else
+      gt.template load<TM, NW * 64, true, FB16>(d, row0, M, rpr, regA, Ds, Cs, dirs, cams);
+  } else if (STD && d.ncam == 0) {
     load_tile_rows<TM, NW * 64, false, FB16>(d, feats, dirs, row0, M, rpr, regA, Ds);
-  else if (STD)
+  } else if (STD) {
     load_tile_rows<TM, NW * 64, true, FB16>(d, feats, dirs, row0, M, rpr, regA, Ds, cams, Cs);
-  else
+  } else {
     load_tile<TM>(d, feats, dirs, cams, row0, M, rpr, regA, Ds, Cs);
+  }
   __syncthreads();
   // layer 1 reads X (regB) and writes H1 over Fs (regA, dead after layer 0);
   // layer 2 reads H1 and writes H2 over X (dead after layer 1)

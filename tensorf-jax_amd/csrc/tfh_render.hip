@@ -27,6 +27,10 @@ int32_t check_mlp(const tfh_mlp* p, const char* who);
 int32_t mlp_fwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
                      int64_t M, int64_t rows_per_ray, float* rgb, int32_t precision, void* ws,
                      size_t ws_bytes, hipStream_t st, float* act = nullptr);
+int32_t mlp_fwd_rows_gather(const tfh_mlp* p, const tfh_vm& vm, const RaySamples& src,
+                            float* feats, const float* dirs, const uint32_t* cams, int64_t M,
+                            int64_t rpr, float* rgb, int32_t precision, void* ws,
+                            size_t ws_bytes, hipStream_t st, float* act, bool* done);
 size_t mlp_bwd_workspace(const tfh_mlp* p, int64_t M, int32_t precision);
 int32_t mlp_bwd_rows(const tfh_mlp* p, const float* feats, const float* dirs, const uint32_t* cams,
                      int64_t M, int64_t rows_per_ray, const float* grgb, float* gfeat,
@@ -673,12 +677,21 @@ static int32_t render_forward(const tfh_vm* density, const tfh_vm* appearance, c
   if (cfg->mode != TFH_MODE_RGB) return TFH_OK;
   const int64_t M = R * k;
   RaySamples src{ws.d.pts, ws.d.idx, k, N};
+  // the standard MLP gathers its feature rows in its own tile load (the
+  // rows also land in ws.feats for the backward)
+  bool fused = false;
   {
-    ProfScope ps("appearance_gather", st);
-    TFH_TRY((launch_vm_gather<RaySamples, true>(*appearance, src, M, ws.feats, st,
-                                                 k >= (int)appearance->grid_dim)));
+    ProfScope ps("mlp_fwd", st);
+    TFH_TRY(mlp_fwd_rows_gather(mlp, *appearance, src, ws.feats, in.d, cams, M, k, ws.rgb,
+                                cfg->mlp_precision, ws.mlp_ws, ws.mlp_ws_bytes, st, ws.act,
+                                &fused));
   }
-  {
+  if (!fused) {
+    {
+      ProfScope ps("appearance_gather", st);
+      TFH_TRY((launch_vm_gather<RaySamples, true>(*appearance, src, M, ws.feats, st,
+                                                   k >= (int)appearance->grid_dim)));
+    }
     ProfScope ps("mlp_fwd", st);
     TFH_TRY(mlp_fwd_rows(mlp, ws.feats, in.d, cams, M, k, ws.rgb, cfg->mlp_precision, ws.mlp_ws,
                          ws.mlp_ws_bytes, st, ws.act));
