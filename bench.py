@@ -77,7 +77,7 @@ def stage_model(stage, R, N, k, cd, ca, precision, n_params, prop=None):
     return None
 
 
-def compulsory_bytes(stage, R, N, k, cd, ca, G, n_params):
+def compulsory_bytes(stage, R, N, k, cd, ca, G, n_params, precision="fp32"):
     """HBM bytes a launch cannot avoid: every input read once, every output
     written once, the factors (or their gradients, read and written) once --
     no tap re-reads. Achieved / 8 TB/s on this model is at most 1 for any
@@ -108,6 +108,13 @@ def compulsory_bytes(stage, R, N, k, cd, ca, G, n_params):
         return R * N * (12 + 12 + 4) + M * 4 * 3 * ca + 3 * (dfac + afac)
     if stage == "adam":
         return n_params * 32
+    if stage == "mlp_fwd_kernel":
+        # the standard MLP forward gathers its own rows: points (and the
+        # selection), view directions and the appearance factors in; the
+        # feature rows (kept for the backward), the hidden activations kept
+        # by a training step (1 KB a row FP32, 512 B BF16) and the colours out
+        kept = 1024 if precision == "fp32" else 512
+        return M * (12 + sel) + R * 12 + afac + M * (4 * 3 * ca + kept + 12)
     return None
 
 
@@ -526,7 +533,7 @@ def main():
                                            f"in this run)") if traffic is not None else None}
         hbm = {}
         for name, (ms, cnt) in rec.items():
-            b = compulsory_bytes(name, R, N, k, cd, ca, G, n_params)
+            b = compulsory_bytes(name, R, N, k, cd, ca, G, n_params, args.precision)
             if b is None:
                 continue
             avg = ms / max(cnt, 1)

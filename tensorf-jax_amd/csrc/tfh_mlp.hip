@@ -188,31 +188,59 @@ struct GatherTile {
     const int G = (int)vm.grid_dim;
     const int64_t row = row0 + i;
     const bool valid = row < M;
+    // the view directions and cameras first, so that their loads overlap
+    // the taps'
+    float dv = 0.f;
+    if (t < TM * 3) {
+      const int i3 = t / 3, c3 = t - 3 * i3;
+      const int64_t r = row0 + i3;
+      if (r < M) dv = dirs[(r / rpr) * 3 + c3];
+    }
+    int cam = 0;
+    if constexpr (CAMS) {
+      if (t < TM) {
+        const int64_t r = row0 + t;
+        if (r < M) cam = (int)cams[r / rpr];
+      }
+    }
     float x = 0.f, y = 0.f, z = 0.f;
     if (valid) src.get(row, x, y, z);
     ModeTaps mt[3];
     mode_taps(x, y, z, G, mt);
+    // the rows go to feats after every mode's taps are loaded: a store to
+    // feats between them would order the next mode's loads behind it (feats
+    // may alias the factors as far as the compiler knows). The 64-register
+    // budget of four workgroups per CU keeps one mode's taps in flight at a
+    // time; holding two took 98 registers (or spilled at 64).
+    float4 f[3];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+      f[m] = make_float4(0.f, 0.f, 0.f, 0.f);
       if (valid && c4 < C) {
         TapAddr a;
         tap_addrs(mt[m], m, G, a);
         float4 lv, pv;
         tap_values4(vm.plane, vm.line, C, c4, a, lv, pv);
         // tensor_vm.py:167
-        f = make_float4(lv.x * pv.x, lv.y * pv.y, lv.z * pv.z, lv.w * pv.w);
-        *reinterpret_cast<float4*>(feats + row * 3 * C + m * C + c4) = f;
+        f[m] = make_float4(lv.x * pv.x, lv.y * pv.y, lv.z * pv.z, lv.w * pv.w);
       }
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
       if (c4 < C) {
         if constexpr (FB16) {
           typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-          const bf16x4 b = {(__bf16)f.x, (__bf16)f.y, (__bf16)f.z, (__bf16)f.w};
+          const bf16x4 b = {(__bf16)f[m].x, (__bf16)f[m].y, (__bf16)f[m].z, (__bf16)f[m].w};
           *reinterpret_cast<bf16x4*>(reinterpret_cast<__bf16*>(Fs) + i * kFsLdB + m * C + c4) = b;
         } else {
-          *reinterpret_cast<float4*>(Fs + i * d.ldF + m * C + c4) = f;
+          *reinterpret_cast<float4*>(Fs + i * d.ldF + m * C + c4) = f[m];
         }
       }
+    }
+    if (valid && c4 < C) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        *reinterpret_cast<float4*>(feats + row * 3 * C + m * C + c4) = f[m];
     }
     if constexpr (FB16) {
       // columns 3C .. FP - 1 (the K padding of layer 0) are zero
@@ -224,17 +252,9 @@ struct GatherTile {
                                    4 * (l - 12)) = zero;
       }
     }
-    if (t < TM * 3) {
-      const int i3 = t / 3, c3 = t - 3 * i3;
-      const int64_t r = row0 + i3;
-      Ds[t] = r < M ? dirs[(r / rpr) * 3 + c3] : 0.0f;
-    }
+    if (t < TM * 3) Ds[t] = dv;
     if constexpr (CAMS) {
-      if (t < TM) {
-        const int64_t r = row0 + t;
-        const int cam = r < M ? (int)cams[r / rpr] : 0;
-        Cs[t] = (cam < 0 || cam >= d.ncam) ? 0 : cam;
-      }
+      if (t < TM) Cs[t] = (cam < 0 || cam >= d.ncam) ? 0 : cam;
     }
   }
 };
