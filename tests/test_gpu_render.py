@@ -170,12 +170,16 @@ def test_distance_modes_closed_form(sigma):
     np.testing.assert_allclose(got["mean"], mean, rtol=2e-5)
 
 
-@pytest.mark.parametrize("n,k,ncam,contraction", [(64, 16, None, False), (48, 48, None, False),
-                                                  (64, 12, 4, True)])
-def test_render_backward(n, k, ncam, contraction):
-    log(f"case: render backward N={n} k={k} ncam={ncam} contraction={contraction}")
+# (127 rays x 13 = 1651 rows: the last 32-row tile of the MLP kernels is
+# partial, with the feature rows written by the forward's own gather)
+@pytest.mark.parametrize("n,k,ncam,contraction,r", [(64, 16, None, False, 128),
+                                                    (48, 48, None, False, 128),
+                                                    (64, 12, 4, True, 128),
+                                                    (64, 13, None, False, 127)])
+def test_render_backward(n, k, ncam, contraction, r):
+    log(f"case: render backward N={n} k={k} ncam={ncam} contraction={contraction} R={r}")
     from tensorf_amd import render, networks
-    shape, p, o, d, cams, box = _setup(r=128, ncam=ncam, contraction=contraction,
+    shape, p, o, d, cams, box = _setup(r=r, ncam=ncam, contraction=contraction,
                                        radius=3.0 if contraction else 4.0)
     lp, rays, mlp = _gpu_objects(shape, p, o, d, cams, contraction)
     leaves = [lp.density_tensor.stacked_single_vm.plane, lp.density_tensor.stacked_single_vm.line,
@@ -279,3 +283,31 @@ def test_render_rgb_bf16_mlp():
                                   shape, jit)
     err = report("rgb (bf16 mlp)", out, want)
     assert err.max() < 2e-2
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_render_rgb_ragged_rows(precision):
+    """191 rays x 13 shaded samples = 2483 rows: the MLP forward, which
+    gathers its own feature rows, ends on a partial 32-row tile whose rows
+    past the end read as zero and are not stored. FP32 against the float64
+    oracle; BF16 against the oracle with BF16-rounded MLP operands."""
+    log(f"case: render rgb ragged rows R=191 N=64 k=13 {precision}")
+    from tensorf_amd import render
+    shape, p, o, d, cams, box = _setup(r=191, seed=3)
+    lp, rays, mlp = _gpu_objects(shape, p, o, d, cams, False, precision=precision)
+    cfg = render.RenderConfig(0.05, 200.0, render.RenderMode.RGB, 64, 13)
+    with torch.no_grad():
+        out = render.render_rays(mlp, lp, dev(box), rays, 9, cfg).cpu().numpy()
+    jit = _oracle_jitter(9, len(o), 64, False)
+    rnd = orc.bf16_round if precision == "bf16" else None
+    want, cache = orc.render_rays(p, box, o, d, cams, orc.RenderConfig(0.05, 200.0, orc.RGB, 64, 13),
+                                  shape, jit, mlp_operand_round=rnd)
+    err = report(f"rgb ragged rows ({precision})", out, want)
+    tol = 2e-3 if precision == "bf16" else 1e-5 * np.abs(want) + 2e-6
+    bad = np.any(err > tol, axis=1)
+    # only rays whose selection sits on a near-tie of the float64 weights
+    w = cache["w"]
+    for r in np.nonzero(bad)[0]:
+        srt = np.sort(w[r])[::-1]
+        assert abs(srt[12] - srt[13]) < 1e-5 * max(srt[12], 1e-30), (r, err[r])
+    assert bad.mean() < 0.02
