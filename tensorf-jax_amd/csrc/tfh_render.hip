@@ -814,7 +814,8 @@ static int32_t check_render_args(const tfh_vm* density, const tfh_vm* appearance
   TFH_TRY(check_cfg(cfg, who));
   TFH_TRY(check_vm(density, who));
   TFH_REQUIRE(R >= 0 && R < (1ll << 31), "%s: num_rays %lld out of range", who, (long long)R);
-  TFH_REQUIRE(aabb && o && d, "%s: null aabb/origins/directions", who);
+  // (an empty batch may pass the null pointers of empty arrays)
+  TFH_REQUIRE(aabb && (R == 0 || (o && d)), "%s: null aabb/origins/directions", who);
   if (cfg->sampling == TFH_SAMPLING_EDGES && samples)
     TFH_REQUIRE(base_ts, "%s: interval-edge sampling needs the edges (base_ts)", who);
   else if (cfg->scene_contraction && samples)

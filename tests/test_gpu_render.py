@@ -285,6 +285,26 @@ def test_render_rgb_bf16_mlp():
     assert err.max() < 2e-2
 
 
+def test_render_zero_rays():
+    """An empty ray batch: the render returns an empty colour batch and its
+    backward leaves every gradient zero."""
+    log("case: render with no rays")
+    from tensorf_amd import render, networks
+    shape, p, o, d, cams, box = _setup(r=4)
+    lp, rays, mlp = _gpu_objects(shape, p, o[:0], d[:0], cams[:0], False)
+    leaves = [lp.density_tensor.stacked_single_vm.plane,
+              lp.appearance_tensor.stacked_single_vm.plane,
+              *networks.FeatureMlp.flat_leaves(lp.appearance_mlp_params)]
+    for t in leaves:
+        t.requires_grad_(True)
+    cfg = render.RenderConfig(0.05, 200.0, render.RenderMode.RGB, 64, 16)
+    out = render.render_rays(mlp, lp, dev(box), rays, 5, cfg)
+    assert tuple(out.shape) == (0, 3)
+    out.sum().backward()
+    for t in leaves:
+        assert t.grad is None or not t.grad.any()
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_render_rgb_ragged_rows(precision):
     """191 rays x 13 shaded samples = 2483 rows: the MLP forward, which
