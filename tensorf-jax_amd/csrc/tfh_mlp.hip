@@ -278,7 +278,7 @@ int32_t mlp_fwd_rows_gather(const tfh_mlp* p, const tfh_vm& vm, const RaySamples
   TFH_TRY(build_pack(p, d, bf, cv, pk, st));
   const size_t lds = fwd_lds_floats<kFwdTM>(d, bf) * sizeof(float);
   TFH_REQUIRE(lds <= 160 * 1024, "mlp fwd: tile needs %zu bytes of LDS", lds);
-  const int64_t blocks = ceil_div(M, kFwdTM);
+  const int64_t blocks = xcd_grid(ceil_div(M, kFwdTM));  // XCD-ordered tiles
   const GatherTile gt{vm, src, feats};
   ProfScope ps("mlp_fwd_kernel", st);
   if (bf) {

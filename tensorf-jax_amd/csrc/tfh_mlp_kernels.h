@@ -611,7 +611,14 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   int* Cs = reinterpret_cast<int*>(Ds + TM * 4);
   float* R3 = regA;
   const Operands<BF> op{d, w, pk};
-  const int64_t row0 = (int64_t)blockIdx.x * TM;
+  int64_t tile = blockIdx.x;
+  if constexpr (kGather) {
+    // XCD-ordered tiles over xcd_grid(tiles) blocks: neighbouring tiles hold
+    // neighbouring samples of a ray, whose taps share texels
+    tile = xcd_unit((M + TM - 1) / TM);
+    if (tile * TM >= M) return;
+  }
+  const int64_t row0 = tile * TM;
   if constexpr (kGather) {
     if (d.ncam == 0)
       gt.template load<TM, NW * 64, false, FB16>(d, row0, M, rpr, regA, Ds, Cs, dirs, cams);
