@@ -108,6 +108,11 @@ def compulsory_bytes(stage, R, N, k, cd, ca, G, n_params, precision="fp32"):
         return R * N * (12 + 12 + 4) + M * 4 * 3 * ca + 3 * (dfac + afac)
     if stage == "adam":
         return n_params * 32
+    if stage == "composite_rgb":
+        # per shaded sample: its weight and colour (and the selection) in; per
+        # ray: the transmittance and unbiasing terms and the target colour
+        # in, the colour and dL/dC out
+        return M * (16 + sel) + R * (8 + 12 + 12 + 12)
     if stage == "mlp_fwd_kernel":
         # the standard MLP forward gathers its own rows: points (and the
         # selection), view directions and the appearance factors in; the
