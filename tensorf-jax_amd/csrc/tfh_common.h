@@ -79,11 +79,11 @@ struct Carve {
 // The hardware deals a grid's blocks to the part's 8 XCDs in turn, each with
 // its own L2. Over a grid of xcd_grid(units) blocks, block b takes unit
 // (b % 8) * per + b / 8 (per = ceil(units / 8)), so that each XCD walks one
-// contiguous range of units: neighbouring units (neighbouring samples of a
-// ray, whose taps share texels) then meet in one L2. Units past the end
+// contiguous range of units instead of every eighth one. Units past the end
 // (at most 7) come out >= units; their blocks return at once. (The MLP
 // forward that gathers its rows: uniform-512 2.30 -> 2.22-2.27 ms, fisheye
-// 5.67 -> 4.99 ms; the ray-order line passes gained nothing from it.)
+// 5.67 -> 4.99 ms, although its L2 hit rate did not rise; the ray-order
+// line passes gained nothing from it.)
 constexpr int kXcds = 8;
 inline int64_t xcd_grid(int64_t units) { return kXcds * ceil_div(units, kXcds); }
 __device__ __forceinline__ int64_t xcd_unit(int64_t units) {

@@ -613,8 +613,8 @@ __global__ void __launch_bounds__(NW * 64) k_mlp_fwd(MlpDims d_arg, MlpW w, MlpP
   const Operands<BF> op{d, w, pk};
   int64_t tile = blockIdx.x;
   if constexpr (kGather) {
-    // XCD-ordered tiles over xcd_grid(tiles) blocks: neighbouring tiles hold
-    // neighbouring samples of a ray, whose taps share texels
+    // XCD-ordered tiles over xcd_grid(tiles) blocks: each XCD walks one
+    // contiguous range of tiles
     tile = xcd_unit((M + TM - 1) / TM);
     if (tile * TM >= M) return;
   }
