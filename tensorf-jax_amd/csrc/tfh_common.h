@@ -83,7 +83,8 @@ struct Carve {
 // (at most 7) come out >= units; their blocks return at once. (The MLP
 // forward that gathers its rows: uniform-512 2.30 -> 2.22-2.27 ms, fisheye
 // 5.67 -> 4.99 ms, although its L2 hit rate did not rise; the ray-order
-// line passes gained nothing from it.)
+// line passes gained nothing from it, and the passes over sorted pairs
+// lost: the proposal-128 line-cell walk 1.33 -> 1.60 ms.)
 constexpr int kXcds = 8;
 inline int64_t xcd_grid(int64_t units) { return kXcds * ceil_div(units, kXcds); }
 __device__ __forceinline__ int64_t xcd_unit(int64_t units) {
